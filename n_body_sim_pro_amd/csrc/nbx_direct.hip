@@ -1,0 +1,228 @@
+// nbx_direct.hip - all-pairs gravitational accelerations on gfx950.
+//
+// direct_fp64: bit-identical to the scalar reference kernel
+//   (src/c/physics/gravity.c:37-67). Every pair term is formed with the
+//   reference's operations in the reference's order (this file is compiled
+//   with -ffp-contract=off, FP64 sqrt and division are IEEE correctly
+//   rounded), and each target's three sums are accumulated in ascending j
+//   with j == i skipped. Only the term formation is parallel: four waves
+//   compute a 64-target x 32-source tile of terms into LDS, then three waves
+//   (one per component) add them in order. This keeps the serial dependency
+//   (one add per term) off the expensive part (div + sqrt per term).
+//
+// direct_fp32: FP32 pair arithmetic on FP32-rounded positions/masses with a
+//   wave-uniform source index, so every source is fetched once per wave by
+//   scalar loads and broadcast through SGPR operands (no LDS traffic). Each
+//   lane owns R targets. Partial sums of T sources are folded into a
+//   per-target compensated (Kahan-Babuska) accumulator once per tile.
+#include "nbx_internal.h"
+
+namespace nbx {
+namespace {
+
+constexpr int kExactTargets = 64;
+constexpr int kExactSources = 32;
+constexpr int kExactThreads = 256;
+
+__global__ __launch_bounds__(kExactThreads) void direct_fp64_kernel(
+    const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
+    const double* __restrict__ m, double* __restrict__ ax, double* __restrict__ ay,
+    double* __restrict__ az, int n, double G, double eps2) {
+    __shared__ double terms[3][kExactSources][kExactTargets];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    const int i = blockIdx.x * kExactTargets + lane;
+    const int ic = i < n ? i : n - 1;
+    const double xi = px[ic], yi = py[ic], zi = pz[ic];
+    double acc = 0.0;
+    for (int j0 = 0; j0 < n; j0 += kExactSources) {
+        // Phase A: each wave forms 8 sources' terms for the block's 64 targets.
+#pragma unroll 2
+        for (int k = 0; k < kExactSources / 4; ++k) {
+            const int jj = wave * (kExactSources / 4) + k;
+            const int j = j0 + jj;
+            const int jc = j < n ? j : n - 1;
+            const double dx = px[jc] - xi;
+            const double dy = py[jc] - yi;
+            const double dz = pz[jc] - zi;
+            const double r2 = dx * dx + dy * dy + dz * dz + eps2;
+            const double inv = 1.0 / sqrt(r2);
+            const double inv3 = inv * inv * inv;
+            const double s = G * m[jc] * inv3;
+            terms[0][jj][lane] = s * dx;
+            terms[1][jj][lane] = s * dy;
+            terms[2][jj][lane] = s * dz;
+        }
+        __syncthreads();
+        // Phase B: in-order sums, one component per wave.
+        if (wave < 3) {
+            const int jn = (n - j0) < kExactSources ? (n - j0) : kExactSources;
+            for (int jj = 0; jj < jn; ++jj) {
+                if (j0 + jj != i) {
+                    acc += terms[wave][jj][lane];
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (i < n && wave < 3) {
+        double* out = wave == 0 ? ax : (wave == 1 ? ay : az);
+        out[i] = acc;
+    }
+}
+
+__device__ inline void two_sum_add(float& s, float& c, float v) {
+    // Kahan-Babuska (Neumaier) step: exact error term of s + v goes to c.
+    const float t = s + v;
+    const float e = (fabsf(s) >= fabsf(v)) ? ((s - t) + v) : ((v - t) + s);
+    s = t;
+    c += e;
+}
+
+template <int R, bool ZERO_EPS>
+__global__ __launch_bounds__(256) void direct_fp32_kernel(const float4* __restrict__ pos,
+                                                          double* __restrict__ ax,
+                                                          double* __restrict__ ay,
+                                                          double* __restrict__ az, int n,
+                                                          float G, float eps2, int tile) {
+    const int base = blockIdx.x * (256 * R) + threadIdx.x;
+    float xi[R], yi[R], zi[R];
+    float sx[R], sy[R], sz[R], cx[R], cy[R], cz[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int i = base + r * 256;
+        const float4 p = pos[i < n ? i : n - 1];
+        xi[r] = p.x;
+        yi[r] = p.y;
+        zi[r] = p.z;
+        sx[r] = sy[r] = sz[r] = 0.f;
+        cx[r] = cy[r] = cz[r] = 0.f;
+    }
+    for (int j0 = 0; j0 < n; j0 += tile) {
+        const int j1 = (j0 + tile) < n ? (j0 + tile) : n;
+        float px_[R], py_[R], pz_[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            px_[r] = py_[r] = pz_[r] = 0.f;
+        }
+#pragma unroll 4
+        for (int j = j0; j < j1; ++j) {
+            const float4 q = pos[j];
+            const float gm = G * q.w;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const float dx = q.x - xi[r];
+                const float dy = q.y - yi[r];
+                const float dz = q.z - zi[r];
+                const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
+                float inv = __builtin_amdgcn_rsqf(r2);
+                if (ZERO_EPS) {
+                    inv = r2 > 0.f ? inv : 0.f;
+                }
+                const float inv2 = inv * inv;
+                const float s = (gm * inv) * inv2;
+                px_[r] = fmaf(s, dx, px_[r]);
+                py_[r] = fmaf(s, dy, py_[r]);
+                pz_[r] = fmaf(s, dz, pz_[r]);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            two_sum_add(sx[r], cx[r], px_[r]);
+            two_sum_add(sy[r], cy[r], py_[r]);
+            two_sum_add(sz[r], cz[r], pz_[r]);
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int i = base + r * 256;
+        if (i < n) {
+            ax[i] = static_cast<double>(sx[r]) + static_cast<double>(cx[r]);
+            ay[i] = static_cast<double>(sy[r]) + static_cast<double>(cy[r]);
+            az[i] = static_cast<double>(sz[r]) + static_cast<double>(cz[r]);
+        }
+    }
+}
+
+__global__ void refresh_pos4_kernel(const double* __restrict__ px, const double* __restrict__ py,
+                                    const double* __restrict__ pz, const double* __restrict__ m,
+                                    float4* __restrict__ out, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        out[i] = make_float4(static_cast<float>(px[i]), static_cast<float>(py[i]),
+                             static_cast<float>(pz[i]), static_cast<float>(m[i]));
+    }
+}
+
+}  // namespace
+
+NBodySimProStatus refresh_pos4(Engine* e, NBodySimProError* error) {
+    if (e->pos4_valid || e->n == 0) {
+        return N_BODY_SIM_PRO_STATUS_OK;
+    }
+    const int n = static_cast<int>(e->n);
+    refresh_pos4_kernel<<<grid_for(e->n, 256), 256, 0, e->stream>>>(e->px, e->py, e->pz, e->m,
+                                                                     e->pos4, n);
+    NBX_HIP(error, hipGetLastError());
+    e->pos4_valid = true;
+    return N_BODY_SIM_PRO_STATUS_OK;
+}
+
+NBodySimProStatus direct_fp64(Engine* e, const NBodySimProGravity* g, NBodySimProError* error) {
+    const size_t n = e->n;
+    if (n == 0) {
+        return N_BODY_SIM_PRO_STATUS_OK;
+    }
+    NBX_HIP(error, hipEventRecord(e->ev[0], e->stream));
+    direct_fp64_kernel<<<grid_for(n, kExactTargets), kExactThreads, 0, e->stream>>>(
+        e->px, e->py, e->pz, e->m, e->ax, e->ay, e->az, static_cast<int>(n),
+        g->gravitational_constant, g->softening_squared);
+    NBX_HIP(error, hipGetLastError());
+    NBX_HIP(error, hipEventRecord(e->ev[1], e->stream));
+    NBX_HIP(error, hipEventSynchronize(e->ev[1]));
+    float ms = 0.f;
+    NBX_HIP(error, hipEventElapsedTime(&ms, e->ev[0], e->ev[1]));
+    e->timings = NBXTimings{};
+    e->timings.force_ms = ms;
+    e->timings.kernel_ms = ms;
+    const double pairs = static_cast<double>(n) * static_cast<double>(n - 1);
+    e->timings.kernel_flops = 20.0 * pairs;
+    e->timings.kernel_bytes = 4.0 * 8.0 * static_cast<double>(n) + 3.0 * 8.0 * static_cast<double>(n);
+    return N_BODY_SIM_PRO_STATUS_OK;
+}
+
+NBodySimProStatus direct_fp32(Engine* e, const NBodySimProGravity* g, NBodySimProError* error) {
+    const size_t n = e->n;
+    if (n == 0) {
+        return N_BODY_SIM_PRO_STATUS_OK;
+    }
+    NBX_HIP(error, hipEventRecord(e->ev[0], e->stream));
+    NBX_TRY(refresh_pos4(e, error));
+    NBX_HIP(error, hipEventRecord(e->ev[1], e->stream));
+    constexpr int R = 2;
+    const unsigned grid = grid_for(n, 256 * R);
+    const float G = static_cast<float>(g->gravitational_constant);
+    const float eps2 = static_cast<float>(g->softening_squared);
+    if (eps2 > 0.f) {
+        direct_fp32_kernel<R, false><<<grid, 256, 0, e->stream>>>(e->pos4, e->ax, e->ay, e->az,
+                                                                  static_cast<int>(n), G, eps2, 256);
+    } else {
+        direct_fp32_kernel<R, true><<<grid, 256, 0, e->stream>>>(e->pos4, e->ax, e->ay, e->az,
+                                                                 static_cast<int>(n), G, eps2, 256);
+    }
+    NBX_HIP(error, hipGetLastError());
+    NBX_HIP(error, hipEventRecord(e->ev[2], e->stream));
+    NBX_HIP(error, hipEventSynchronize(e->ev[2]));
+    float total = 0.f, kern = 0.f;
+    NBX_HIP(error, hipEventElapsedTime(&total, e->ev[0], e->ev[2]));
+    NBX_HIP(error, hipEventElapsedTime(&kern, e->ev[1], e->ev[2]));
+    e->timings = NBXTimings{};
+    e->timings.force_ms = total;
+    e->timings.kernel_ms = kern;
+    const double pairs = static_cast<double>(n) * static_cast<double>(n - 1);
+    e->timings.kernel_flops = 20.0 * pairs;
+    e->timings.kernel_bytes = 16.0 * static_cast<double>(n) + 24.0 * static_cast<double>(n);
+    return N_BODY_SIM_PRO_STATUS_OK;
+}
+
+}  // namespace nbx

@@ -1,0 +1,172 @@
+// nbx_internal.h - engine state and helpers shared by the HIP translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+
+#include "nbx_engine.h"
+
+namespace nbx {
+
+constexpr int kWave = 64;
+
+// Fills the optional error record (reference contract: core/status.h:37-49).
+void set_error(NBodySimProError* error, NBodySimProStatus status, const char* file, int line,
+               const char* message);
+NBodySimProStatus map_hip_error(hipError_t e);
+
+#define NBX_FAIL(err, st, msg)                                 \
+    do {                                                       \
+        ::nbx::set_error((err), (st), __FILE__, __LINE__, (msg)); \
+        return (st);                                           \
+    } while (0)
+
+#define NBX_HIP(err, expr)                                                        \
+    do {                                                                          \
+        hipError_t nbx_e_ = (expr);                                               \
+        if (nbx_e_ != hipSuccess) {                                               \
+            NBodySimProStatus nbx_s_ = ::nbx::map_hip_error(nbx_e_);              \
+            char nbx_m_[200];                                                     \
+            std::snprintf(nbx_m_, sizeof(nbx_m_), "%s: %s", #expr,                \
+                          hipGetErrorString(nbx_e_));                             \
+            ::nbx::set_error((err), nbx_s_, __FILE__, __LINE__, nbx_m_);          \
+            return nbx_s_;                                                        \
+        }                                                                         \
+    } while (0)
+
+#define NBX_TRY(expr)                               \
+    do {                                            \
+        NBodySimProStatus nbx_t_ = (expr);          \
+        if (nbx_t_ != N_BODY_SIM_PRO_STATUS_OK) {   \
+            return nbx_t_;                          \
+        }                                           \
+    } while (0)
+
+// Grows a device allocation to at least `bytes` (contents not preserved).
+NBodySimProStatus ensure(void** ptr, size_t* cap, size_t bytes, NBodySimProError* error);
+template <typename T>
+inline NBodySimProStatus ensure_n(T** ptr, size_t* cap, size_t count, NBodySimProError* error) {
+    return ensure(reinterpret_cast<void**>(ptr), cap, count * sizeof(T), error);
+}
+
+struct Buf {
+    void* p = nullptr;
+    size_t cap = 0;
+    template <typename T>
+    T* as() const { return static_cast<T*>(p); }
+    template <typename T>
+    NBodySimProStatus need(size_t count, NBodySimProError* error) {
+        return ensure(&p, &cap, count * sizeof(T), error);
+    }
+};
+
+// FP32 walk node: centre of mass, mass, opening size^2 (4h^2), pre-order skip, leaf.
+struct alignas(32) NodeF32 {
+    float cx, cy, cz, m;
+    float size2;
+    int32_t skip;
+    int32_t leaf;   // tree position of the leaf particle, -1 for an internal node
+    int32_t depth;
+};
+
+// FP64 walk node (exact reference arithmetic).
+struct alignas(16) NodeF64 {
+    double cx, cy, cz, m;
+    double h;       // cell half size at this depth
+    int32_t skip;
+    int32_t leaf;
+};
+
+// Barnes-Hut device workspace (all sized by particle or node capacity).
+struct BHState {
+    size_t n = 0;             // particles in the last build
+    size_t internal = 0;      // internal nodes in the last build
+    size_t nodes = 0;         // n + internal
+    int max_depth = 0;
+    bool tie_path = false;
+    bool mismatch_path = false;
+    double root[4] = {0, 0, 0, 1};
+    // build inputs (FP64 x, y, z, m): either the master arrays or FP32-rounded copies
+    Buf bx, by, bz, bm;
+    Buf partial;              // bbox partials + root
+    Buf counter;              // block-done counters
+    Buf flags;                // [0] mismatch, [1] tie, [2] overflow, [3] max depth
+    Buf mkey, tkey;           // Morton / true-path keys in original order
+    Buf sk0, sk1, sv0, sv1;   // sort ping-pong
+    Buf hist;                 // sort histograms
+    Buf mperm;                // Morton order: sorted position -> original index
+    Buf mkey_sorted;          // Morton keys ascending
+    Buf tperm;                // tree position -> original index
+    Buf trank;                // tree position -> Morton rank
+    Buf tword;                // tree-order true keys (descending)
+    Buf ext;                  // extension key words (6 per particle) on the tie path
+    Buf delta;                // int32 common levels between tree positions t, t+1
+    Buf cnt, scan;            // internal nodes started per tree position, exclusive scan
+    Buf scan_part;            // scan block partials
+    Buf sx, sy, sz, sm;       // tree-order FP64 particle data
+    Buf s4;                   // tree-order FP32 x, y, z, m
+    // nodes (pre-order, children in descending octant order)
+    Buf n64;                  // NodeF64
+    Buf n32;                  // NodeF32
+    Buf ncx, ncy, ncz, nm;    // FP64 centre of mass and mass per node
+    Buf nskip, nleaf, nparent, nchild, narrive, nstart, ncount, ndepth, nminpos;
+    Buf tmp32;                // scratch permutation
+    Buf counters;             // unsigned long long [approx, exact]
+};
+
+struct Engine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    size_t capacity = 0;
+    size_t n = 0;
+    double theta = 0.7;
+    // FP64 master state (SoA)
+    double *px = nullptr, *py = nullptr, *pz = nullptr;
+    double *vx = nullptr, *vy = nullptr, *vz = nullptr;
+    double *ax = nullptr, *ay = nullptr, *az = nullptr;
+    double* m = nullptr;
+    float4* pos4 = nullptr;   // FP32 (x, y, z, m) mirror of the master
+    bool pos4_valid = false;
+    Buf scratch;              // diagnostics / misc
+    Buf scratch2;
+    BHState bh;
+    NodeF32* learned_dummy = nullptr;
+    NBodySimProBarnesHutStats stats{};
+    NBXTimings timings{};
+    hipEvent_t ev[12]{};
+    // learned BH correction
+    void* mlp = nullptr;
+};
+
+// ---- entry points implemented per translation unit ----
+NBodySimProStatus direct_fp64(Engine* e, const NBodySimProGravity* g, NBodySimProError* error);
+NBodySimProStatus direct_fp32(Engine* e, const NBodySimProGravity* g, NBodySimProError* error);
+NBodySimProStatus refresh_pos4(Engine* e, NBodySimProError* error);
+NBodySimProStatus barnes_hut(Engine* e, const NBodySimProGravity* g, bool fp64_exact,
+                             NBodySimProError* error);
+NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error);
+NBodySimProStatus bh_walk(Engine* e, const NBodySimProGravity* g, bool fp64_exact,
+                          float* features, NBodySimProError* error);
+NBodySimProStatus bh_learned(Engine* e, const NBodySimProGravity* g, NBodySimProError* error);
+
+// Stable LSD radix sort of (64-bit key, 32-bit value) pairs, ascending key;
+// equal keys keep their input order. `bits` = number of low key bits to sort.
+NBodySimProStatus radix_sort_pairs(hipStream_t s, uint64_t* keys, uint32_t* vals,
+                                   uint64_t* keys_alt, uint32_t* vals_alt, size_t n, int bits,
+                                   Buf& hist, NBodySimProError* error);
+// Exclusive scan of uint32 (total written to out[n]).
+NBodySimProStatus exclusive_scan_u32(hipStream_t s, const uint32_t* in, uint32_t* out, size_t n,
+                                     Buf& partial, NBodySimProError* error);
+
+inline unsigned grid_for(size_t n, unsigned block) {
+    return static_cast<unsigned>((n + block - 1) / block);
+}
+
+}  // namespace nbx
+
+struct NBXEngine {
+    nbx::Engine e;
+};

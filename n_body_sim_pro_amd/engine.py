@@ -1,0 +1,311 @@
+"""Python host layer over the C ABI in include/nbx_engine.h (libnbx.so).
+
+Mirrors the reference's force-kernel plug point: ``force_function(name)``
+returns the C function pointer with the NBodySimProForceFunction signature
+(reference: include/n_body_sim_pro/physics/integrator.h:47-49), and ``Engine``
+wraps the device-resident API. No fallback: when libnbx.so is missing or no
+HIP device is visible, the calls raise.
+"""
+import ctypes as C
+import pathlib
+
+import numpy as np
+
+from .abi import (BarnesHutStats, Diagnostics, EngineConfig, Error, Gravity, Timings, View,
+                  STATUS_NAMES)
+
+PKG = pathlib.Path(__file__).resolve().parent
+LIB_PATH = PKG / "libnbx.so"
+
+ALGORITHMS = {
+    "hip_direct_fp64": 0,
+    "hip_direct_fp32": 1,
+    "hip_barnes_hut": 2,
+    "hip_barnes_hut_fp64": 3,
+    "hip_bh_learned": 4,
+}
+_FORCE_SYMBOLS = {
+    "hip_direct_fp64": "nbx_hip_direct_fp64",
+    "hip_direct_fp32": "nbx_hip_direct_fp32",
+    "hip_barnes_hut": "nbx_hip_barnes_hut",
+    "hip_barnes_hut_fp64": "nbx_hip_barnes_hut_fp64",
+    "hip_bh_learned": "nbx_hip_bh_learned",
+}
+
+_lib = None
+
+
+class NBXError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+def load():
+    """Loads libnbx.so (built in-tree by n_body_sim_pro_amd.build)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} is missing: run `python -m n_body_sim_pro_amd.build`")
+    lib = C.CDLL(str(LIB_PATH))
+    vp, gp, ep = C.POINTER(View), C.POINTER(Gravity), C.POINTER(Error)
+    lib.nbx_engine_create.argtypes = [C.POINTER(EngineConfig), ep]
+    lib.nbx_engine_create.restype = C.c_void_p
+    lib.nbx_engine_destroy.argtypes = [C.c_void_p]
+    lib.nbx_engine_set_theta.argtypes = [C.c_void_p, C.c_double]
+    lib.nbx_engine_theta.argtypes = [C.c_void_p]
+    lib.nbx_engine_theta.restype = C.c_double
+    lib.nbx_engine_upload.argtypes = [C.c_void_p, vp, ep]
+    lib.nbx_engine_download.argtypes = [C.c_void_p, vp, ep]
+    lib.nbx_engine_compute_force.argtypes = [C.c_void_p, C.c_int, gp, ep]
+    lib.nbx_engine_advance.argtypes = [C.c_void_p, C.c_int, gp, C.c_int, C.c_double, C.c_int, ep]
+    lib.nbx_engine_diagnostics.argtypes = [C.c_void_p, gp, C.POINTER(Diagnostics),
+                                           C.POINTER(C.c_double), ep]
+    lib.nbx_engine_stats.argtypes = [C.c_void_p, C.POINTER(BarnesHutStats)]
+    lib.nbx_engine_timings.argtypes = [C.c_void_p, C.POINTER(Timings)]
+    lib.nbx_engine_bh_morton.argtypes = [C.c_void_p, C.POINTER(C.c_double),
+                                         C.POINTER(C.c_uint64), C.POINTER(C.c_int64), ep]
+    lib.nbx_engine_bh_export_nodes.argtypes = [C.c_void_p, C.POINTER(C.c_double), ep]
+    lib.nbx_algorithm_from_name.argtypes = [C.c_char_p]
+    lib.nbx_algorithm_name.argtypes = [C.c_int]
+    lib.nbx_algorithm_name.restype = C.c_char_p
+    lib.nbx_device_count.restype = C.c_int
+    for sym in _FORCE_SYMBOLS.values():
+        getattr(lib, sym).argtypes = [vp, gp, C.c_void_p, ep]
+        getattr(lib, sym).restype = C.c_int
+    _lib = lib
+    return lib
+
+
+def gravity(G=1.0, eps=0.01):
+    g = Gravity()
+    g.gravitational_constant = G
+    g.softening_length = eps
+    g.softening_squared = eps * eps
+    return g
+
+
+def force_function(name):
+    """C function pointer for the drop-in force function `name` (hip_*)."""
+    lib = load()
+    return getattr(lib, _FORCE_SYMBOLS[name])
+
+
+def _check(status, err):
+    if status != 0:
+        raise NBXError(status, err.message.decode(errors="replace"))
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def make_view(n, px=None, py=None, pz=None, vx=None, vy=None, vz=None, ax=None, ay=None,
+              az=None, m=None):
+    """A View over float64 numpy arrays (None fields stay NULL). Keep the arrays alive."""
+    v = View()
+    v.particle_count = int(n)
+    for field, arr in (("positions_x", px), ("positions_y", py), ("positions_z", pz),
+                       ("velocities_x", vx), ("velocities_y", vy), ("velocities_z", vz),
+                       ("accelerations_x", ax), ("accelerations_y", ay), ("accelerations_z", az),
+                       ("masses", m)):
+        if arr is not None:
+            assert arr.dtype == np.float64 and arr.flags.c_contiguous and arr.size >= n
+            setattr(v, field, _ptr(arr))
+    return v
+
+
+class Engine:
+    """Device-resident engine (one HIP stream and its buffers on one device)."""
+
+    def __init__(self, capacity, device=0, theta=0.7):
+        self.lib = load()
+        cfg = EngineConfig(capacity=int(capacity), device=int(device), theta=float(theta), flags=0)
+        err = Error()
+        self.h = self.lib.nbx_engine_create(C.byref(cfg), C.byref(err))
+        if not self.h:
+            raise NBXError(err.status, err.message.decode(errors="replace"))
+        self.capacity = int(capacity)
+        self.n = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.nbx_engine_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def theta(self):
+        return self.lib.nbx_engine_theta(self.h)
+
+    @theta.setter
+    def theta(self, value):
+        self.lib.nbx_engine_set_theta(self.h, float(value))
+
+    def upload_view(self, view):
+        err = Error()
+        _check(self.lib.nbx_engine_upload(self.h, C.byref(view), C.byref(err)), err)
+        self.n = int(view.particle_count)
+
+    def download_view(self, view):
+        err = Error()
+        _check(self.lib.nbx_engine_download(self.h, C.byref(view), C.byref(err)), err)
+
+    def upload(self, pos, mass, vel=None, acc=None):
+        """pos/vel/acc: (n, 3) arrays; mass: (n,)."""
+        n = len(mass)
+        cols = {}
+        p = np.ascontiguousarray(np.asarray(pos, dtype=np.float64).T)
+        cols.update(px=p[0].copy(), py=p[1].copy(), pz=p[2].copy())
+        cols["m"] = np.ascontiguousarray(mass, dtype=np.float64).copy()
+        if vel is not None:
+            v = np.ascontiguousarray(np.asarray(vel, dtype=np.float64).T)
+            cols.update(vx=v[0].copy(), vy=v[1].copy(), vz=v[2].copy())
+        if acc is not None:
+            a = np.ascontiguousarray(np.asarray(acc, dtype=np.float64).T)
+            cols.update(ax=a[0].copy(), ay=a[1].copy(), az=a[2].copy())
+        self.upload_view(make_view(n, **cols))
+
+    def download(self):
+        """Returns dict with pos, vel, acc (n, 3) and m (n,)."""
+        n = self.n
+        cols = {k: np.zeros(n) for k in ("px", "py", "pz", "vx", "vy", "vz", "ax", "ay", "az",
+                                         "m")}
+        self.download_view(make_view(n, **cols))
+        return {"pos": np.stack([cols["px"], cols["py"], cols["pz"]], 1),
+                "vel": np.stack([cols["vx"], cols["vy"], cols["vz"]], 1),
+                "acc": np.stack([cols["ax"], cols["ay"], cols["az"]], 1), "m": cols["m"]}
+
+    def accelerations(self):
+        n = self.n
+        cols = {k: np.zeros(n) for k in ("ax", "ay", "az")}
+        self.download_view(make_view(n, **cols))
+        return np.stack([cols["ax"], cols["ay"], cols["az"]], 1)
+
+    def compute_force(self, algorithm, G=1.0, eps=0.01):
+        algo = ALGORITHMS[algorithm] if isinstance(algorithm, str) else int(algorithm)
+        err = Error()
+        g = gravity(G, eps)
+        _check(self.lib.nbx_engine_compute_force(self.h, algo, C.byref(g), C.byref(err)), err)
+
+    def advance(self, algorithm, dt, steps, G=1.0, eps=0.01, integrator=1):
+        algo = ALGORITHMS[algorithm] if isinstance(algorithm, str) else int(algorithm)
+        err = Error()
+        g = gravity(G, eps)
+        _check(self.lib.nbx_engine_advance(self.h, algo, C.byref(g), int(integrator), float(dt),
+                                           int(steps), C.byref(err)), err)
+
+    def diagnostics(self, G=1.0, eps=0.01, potential=True):
+        q = Diagnostics()
+        u = C.c_double(0.0)
+        err = Error()
+        g = gravity(G, eps)
+        _check(self.lib.nbx_engine_diagnostics(self.h, C.byref(g), C.byref(q),
+                                               C.byref(u) if potential else None, C.byref(err)),
+               err)
+        out = {f: getattr(q, f) for f, _ in q._fields_}
+        if potential:
+            out["potential_energy"] = u.value
+        return out
+
+    def stats(self):
+        s = BarnesHutStats()
+        self.lib.nbx_engine_stats(self.h, C.byref(s))
+        return {f: getattr(s, f) for f, _ in s._fields_}
+
+    def timings(self):
+        t = Timings()
+        self.lib.nbx_engine_timings(self.h, C.byref(t))
+        return {f: getattr(t, f) for f, _ in t._fields_}
+
+    def bh_morton(self):
+        n = self.n
+        root = np.zeros(4)
+        keys = np.zeros(n, dtype=np.uint64)
+        perm = np.zeros(n, dtype=np.int64)
+        err = Error()
+        _check(self.lib.nbx_engine_bh_morton(self.h, root.ctypes.data_as(C.POINTER(C.c_double)),
+                                             keys.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                             perm.ctypes.data_as(C.POINTER(C.c_int64)),
+                                             C.byref(err)), err)
+        return root, keys, perm
+
+    def bh_export_nodes(self):
+        count = self.stats()["node_count"]
+        rows = np.zeros((count, 11))
+        err = Error()
+        _check(self.lib.nbx_engine_bh_export_nodes(
+            self.h, rows.ctypes.data_as(C.POINTER(C.c_double)), C.byref(err)), err)
+        return rows
+
+
+def device_count():
+    return int(load().nbx_device_count())
+
+
+class GalaxyPairParams(C.Structure):
+    """NBXGalaxyPairParams in include/nbx_presets.h."""
+    _fields_ = [(n, C.c_double) for n in (
+        "disk_scale_length", "disk_scale_height", "bulge_fraction", "bulge_scale",
+        "separation", "pericenter", "inclination", "velocity_dispersion")]
+
+
+def _generate(symbol, n, *args):
+    lib = load()
+    cols = {k: np.zeros(int(n)) for k in ("px", "py", "pz", "vx", "vy", "vz", "m")}
+    v = make_view(n, **cols)
+    fn = getattr(lib, symbol)
+    fn.restype = C.c_int
+    st = fn(C.byref(v), *args)
+    if st != 0:
+        raise NBXError(st, f"{symbol} failed")
+    return {"pos": np.stack([cols["px"], cols["py"], cols["pz"]], 1),
+            "vel": np.stack([cols["vx"], cols["vy"], cols["vz"]], 1), "m": cols["m"]}
+
+
+def random_cloud(n, seed=42):
+    """The reference's random_cloud preset (bit-identical)."""
+    return _generate("nbx_preset_random_cloud", n, C.c_uint64(seed))
+
+
+def plummer(n, seed=42, scale_radius=1.0, total_mass=1.0):
+    """Plummer sphere with velocities from the distribution function."""
+    return _generate("nbx_generate_plummer", n, C.c_double(scale_radius),
+                     C.c_double(total_mass), C.c_uint64(seed))
+
+
+def galaxy_pair_defaults():
+    p = GalaxyPairParams()
+    load().nbx_galaxy_pair_defaults(C.byref(p))
+    return p
+
+
+def two_galaxy(n, seed=42, params=None):
+    """Two disk+bulge galaxies on a parabolic approach, 20 units apart."""
+    return _generate("nbx_generate_two_galaxy", n,
+                     C.byref(params) if params is not None else None, C.c_uint64(seed))
+
+
+def disk_galaxy(n, seed=42, params=None):
+    return _generate("nbx_generate_disk_galaxy", n,
+                     C.byref(params) if params is not None else None, C.c_uint64(seed))
+
+
+def ensemble_sizes(clusters, lo=64, hi=4096, seed=42):
+    out = np.zeros(int(clusters), dtype=np.uint32)
+    lib = load()
+    lib.nbx_ensemble_sizes.restype = C.c_int
+    st = lib.nbx_ensemble_sizes(C.c_size_t(clusters), C.c_uint32(lo), C.c_uint32(hi),
+                                C.c_uint64(seed), out.ctypes.data_as(C.POINTER(C.c_uint32)))
+    if st != 0:
+        raise NBXError(st, "nbx_ensemble_sizes failed")
+    return out

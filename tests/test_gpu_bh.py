@@ -1,0 +1,190 @@
+"""Barnes-Hut parity on the GPU: Morton keys, sorted order, root cube and tree
+topology bit-exact against the oracle; hip_barnes_hut_fp64 accelerations and
+statistics bit-identical to the reference (committed vectors and oracle);
+hip_barnes_hut (FP32 walk on FP32-rounded inputs) close to the FP64 result."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from nbx_testlib import BarnesHutStats, Particles, gravity, rms_rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def fixture_particles(golden, name):
+    n = golden[f"{name}/px"].size
+    return Particles(n, {k: golden[f"{name}/{k}"] for k in ("px", "py", "pz", "vx", "vy", "vz", "m")})
+
+
+def from_gen(d, fp32=False):
+    pos, m = d["pos"], d["m"]
+    if fp32:
+        pos = pos.astype(np.float32).astype(np.float64)
+        m = m.astype(np.float32).astype(np.float64)
+    return Particles(len(m), {"px": pos[:, 0], "py": pos[:, 1], "pz": pos[:, 2], "m": m})
+
+
+def oracle_bh(oracle, p, theta, eps):
+    q = p.copy()
+    t = oracle.nbo_bh_create()
+    oracle.nbo_bh_set_theta(t, theta)
+    st = oracle.nbo_bh_compute(t, C.byref(q.view), C.byref(gravity(1.0, eps)))
+    s = BarnesHutStats()
+    oracle.nbo_bh_stats(t, C.byref(s))
+    keys = np.zeros(p.n, dtype=np.uint64)
+    perm = np.zeros(p.n, dtype=np.int64)
+    root = np.zeros(4)
+    rows = None
+    if st == 0:
+        oracle.nbo_bh_keys(t, keys.ctypes.data_as(C.POINTER(C.c_uint64)),
+                           perm.ctypes.data_as(C.POINTER(C.c_int64)))
+        oracle.nbo_bh_root(t, root.ctypes.data_as(C.POINTER(C.c_double)))
+        rows = np.zeros((oracle.nbo_bh_node_count(t), 11))
+        oracle.nbo_bh_export_nodes(t, rows.ctypes.data_as(C.POINTER(C.c_double)))
+    oracle.nbo_bh_destroy(t)
+    return st, q.acc(), s, keys, perm, root, rows
+
+
+def gpu_bh(nbx, p, theta, eps, algorithm):
+    e = nbx.Engine(p.n, theta=theta)
+    e.upload_view(p.view)
+    e.compute_force(algorithm, eps=eps)
+    return e, e.accelerations(), e.stats()
+
+
+def sort_rows(rows):
+    keys = tuple(rows[:, k] for k in (10, 3, 2, 1, 0))
+    return rows[np.lexsort(keys)]
+
+
+def stats_tuple(s):
+    if isinstance(s, dict):
+        return (s["node_count"], s["leaf_count"], s["internal_node_count"],
+                s["accepted_approximations"], s["exact_interactions"], s["maximum_depth"])
+    return (s.node_count, s.leaf_count, s.internal_node_count, s.accepted_approximations,
+            s.exact_interactions, s.maximum_depth)
+
+
+@pytest.mark.parametrize("theta", [0.1, 0.3, 0.7, 1.0, 1.2])
+def test_bh_fp64_bit_identical_to_reference_vectors(nbx, golden, theta):
+    p = fixture_particles(golden, "cloud400_s7")
+    e, acc, s = gpu_bh(nbx, p, theta, 0.02, "hip_barnes_hut_fp64")
+    e.close()
+    assert np.array_equal(acc, golden[f"cloud400_s7/bh_acc_theta{theta}"])
+    assert list(stats_tuple(s)) == list(golden[f"cloud400_s7/bh_stats_theta{theta}"])
+
+
+@pytest.mark.parametrize("name,eps", [("spiral400_s99", 0.05), ("cloud2048_s42", 0.01)])
+def test_bh_fp64_other_reference_vectors(nbx, golden, name, eps):
+    p = fixture_particles(golden, name)
+    e, acc, s = gpu_bh(nbx, p, 0.7, eps, "hip_barnes_hut_fp64")
+    e.close()
+    assert np.array_equal(acc, golden[f"{name}/bh_acc_theta0.7"])
+    assert list(stats_tuple(s)) == list(golden[f"{name}/bh_stats_theta0.7"])
+
+
+def crafted_inputs():
+    rng = np.random.default_rng(3)
+    out = {}
+    # lattice: many particles exactly on cell-centre planes (Morton/octant ties)
+    g = np.arange(-4, 5, dtype=np.float64)
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    pos = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+    out["lattice"] = (pos, np.ones(len(pos)))
+    # pairs closer than one 21-bit Morton cell (tie path: deeper than 21 levels)
+    base = rng.uniform(-1, 1, size=(300, 3))
+    close = base + rng.uniform(-1, 1, size=(300, 3)) * 1e-9
+    pos = np.concatenate([base, close, rng.uniform(-1, 1, size=(400, 3))])
+    out["close_pairs"] = (pos, rng.uniform(0.5, 1.0, len(pos)))
+    # a cluster + far outlier (deep chains of single-child cells)
+    pos = np.concatenate([rng.normal(scale=1e-4, size=(500, 3)), [[100.0, -50.0, 20.0]]])
+    out["outlier"] = (pos, np.ones(len(pos)))
+    return out
+
+
+@pytest.mark.parametrize("case", ["lattice", "close_pairs", "outlier"])
+@pytest.mark.parametrize("theta", [0.3, 0.7])
+def test_bh_fp64_crafted_inputs_bit_identical(nbx, oracle, case, theta):
+    pos, m = crafted_inputs()[case]
+    p = Particles(len(m), {"px": pos[:, 0], "py": pos[:, 1], "pz": pos[:, 2], "m": m})
+    st, oacc, os_, okeys, operm, oroot, orows = oracle_bh(oracle, p, theta, 0.01)
+    assert st == 0
+    e, acc, s = gpu_bh(nbx, p, theta, 0.01, "hip_barnes_hut_fp64")
+    root, keys, perm = e.bh_morton()
+    rows = e.bh_export_nodes()
+    e.close()
+    assert np.array_equal(root, oroot)
+    assert np.array_equal(keys, okeys)
+    assert np.array_equal(perm, operm)
+    assert stats_tuple(s) == stats_tuple(os_)
+    assert np.array_equal(sort_rows(rows), sort_rows(orows))
+    assert np.array_equal(acc, oacc)
+
+
+def test_bh_two_galaxy_fp32_inputs_keys_order_topology(nbx, oracle):
+    """Config 2 shape (scaled to 200k): FP32-rounded positions/masses; the
+    63-bit keys (and hence key >> 33), sorted order and tree are bit-exact."""
+    d = nbx.two_galaxy(200_000, seed=42)
+    p = from_gen(d, fp32=True)
+    st, oacc, os_, okeys, operm, oroot, orows = oracle_bh(oracle, p, 0.7, 0.01)
+    assert st == 0
+    raw = from_gen(d, fp32=False)
+    e, acc, s = gpu_bh(nbx, raw, 0.7, 0.01, "hip_barnes_hut")
+    root, keys, perm = e.bh_morton()
+    rows = e.bh_export_nodes()
+    e.close()
+    assert np.array_equal(root, oroot)
+    assert np.array_equal(keys, okeys)
+    assert np.array_equal(keys >> np.uint64(33), okeys >> np.uint64(33))
+    assert np.array_equal(perm, operm)
+    s_t, o_t = stats_tuple(s), stats_tuple(os_)
+    assert s_t[:3] == o_t[:3] and s_t[5] == o_t[5]
+    assert np.array_equal(sort_rows(rows)[:, :7], sort_rows(orows)[:, :7])
+    # FP32 walk vs the FP64 reference walk on identical inputs
+    err = rms_rel_err(acc, oacc)
+    print(f"fp32 walk rms rel err {err:.3e}; interactions gpu {s_t[3] + s_t[4]} cpu {o_t[3] + o_t[4]}")
+    assert err < 1e-4
+    assert abs((s_t[3] + s_t[4]) - (o_t[3] + o_t[4])) <= 1e-3 * (o_t[3] + o_t[4])
+
+
+def test_bh_fp64_two_galaxy_bit_identical(nbx, oracle):
+    d = nbx.two_galaxy(30_000, seed=5)
+    p = from_gen(d)
+    st, oacc, os_, *_ = oracle_bh(oracle, p, 0.7, 0.01)
+    e, acc, s = gpu_bh(nbx, p, 0.7, 0.01, "hip_barnes_hut_fp64")
+    e.close()
+    assert np.array_equal(acc, oacc)
+    assert stats_tuple(s) == stats_tuple(os_)
+
+
+def test_bh_single_particle_and_pair(nbx, oracle):
+    for pos, m in [([[0.3, -0.2, 0.1]], [2.0]), ([[-0.5, 0, 0], [0.5, 0, 0]], [1.0, 1.0])]:
+        pos = np.array(pos, dtype=np.float64)
+        p = Particles(len(m), {"px": pos[:, 0], "py": pos[:, 1], "pz": pos[:, 2], "m": m})
+        st, oacc, os_, *_ = oracle_bh(oracle, p, 0.7, 0.02)
+        e, acc, s = gpu_bh(nbx, p, 0.7, 0.02, "hip_barnes_hut_fp64")
+        e.close()
+        assert np.array_equal(acc, oacc)
+        assert stats_tuple(s) == stats_tuple(os_)
+
+
+def test_bh_duplicate_positions_overflow(nbx, oracle):
+    pos = np.array([[0.1, 0.2, 0.3]] * 2 + [[1.0, 1.0, 1.0]])
+    p = Particles(3, {"px": pos[:, 0], "py": pos[:, 1], "pz": pos[:, 2], "m": np.ones(3)})
+    st, *_ = oracle_bh(oracle, p, 0.7, 0.01)
+    assert st == 6  # OVERFLOW (barnes_hut.c:149-153)
+    e = nbx.Engine(3)
+    e.upload_view(p.view)
+    with pytest.raises(nbx.NBXError) as ex:
+        e.compute_force("hip_barnes_hut_fp64", eps=0.01)
+    assert ex.value.status == 6
+    e.close()
+
+
+def test_bh_empty_system_is_invalid(nbx):
+    e = nbx.Engine(4)
+    with pytest.raises(nbx.NBXError) as ex:
+        e.compute_force("hip_barnes_hut", eps=0.01)
+    assert ex.value.status == 1
+    e.close()
