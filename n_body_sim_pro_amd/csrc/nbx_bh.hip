@@ -49,7 +49,7 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kExtWords = 6;
 constexpr int kMaxLevels = 129;  // a shared depth-129 cell means depth > 128 (barnes_hut.c:149-153)
-enum { FLAG_MISMATCH = 0, FLAG_TIE = 1, FLAG_OVERFLOW = 2, FLAG_MAXDEPTH = 3, FLAG_COUNT = 8 };
+enum { FLAG_MISMATCH = 0, FLAG_TIE = 1, FLAG_OVERFLOW = 2 };
 
 __device__ inline uint64_t spread3(uint64_t a) {
     uint64_t x = a & 0x1FFFFFull;
@@ -113,9 +113,33 @@ __device__ inline int common_levels(const uint64_t* __restrict__ w,
     return kMaxLevels;
 }
 
-// Last position e >= t with common_levels(t, e) >= d.
+// Octant digit of tree position t at level `level` (the octant of its depth
+// level+1 cell inside its depth `level` cell).
+__device__ inline int digit_at(const uint64_t* __restrict__ w, const uint64_t* __restrict__ ext,
+                               int t, int level) {
+    const uint64_t word = level < 21 ? w[t]
+                                     : ext[static_cast<size_t>(t) * kExtWords + (level / 21 - 1)];
+    return static_cast<int>((word >> (60 - 3 * (level % 21))) & 7u);
+}
+
+// Last position e >= t with common_levels(t, e) >= d (galloping, then binary
+// search: deep cells are short runs, so most searches end after a few probes).
 __device__ inline int run_end(const uint64_t* w, const uint64_t* ext, int n, int t, int d) {
-    int lo = t, hi = n - 1;
+    int lo = t, step = 1;
+    int hi = n - 1;
+    while (true) {
+        const int probe = t + step;
+        if (probe >= n) {
+            break;
+        }
+        if (common_levels(w, ext, t, probe) >= d) {
+            lo = probe;
+            step <<= 1;
+        } else {
+            hi = probe - 1;
+            break;
+        }
+    }
     while (lo < hi) {
         const int mid = lo + (hi - lo + 1) / 2;
         if (common_levels(w, ext, t, mid) >= d) {
@@ -127,9 +151,23 @@ __device__ inline int run_end(const uint64_t* w, const uint64_t* ext, int n, int
     return lo;
 }
 
-// First position s <= t with common_levels(s, t) >= d.
+// First position s <= t with common_levels(s, t) >= d (galloping backwards).
 __device__ inline int run_start(const uint64_t* w, const uint64_t* ext, int t, int d) {
-    int lo = 0, hi = t;
+    int hi = t, step = 1;
+    int lo = 0;
+    while (true) {
+        const int probe = t - step;
+        if (probe < 0) {
+            break;
+        }
+        if (common_levels(w, ext, probe, t) >= d) {
+            hi = probe;
+            step <<= 1;
+        } else {
+            lo = probe + 1;
+            break;
+        }
+    }
     while (lo < hi) {
         const int mid = lo + (hi - lo) / 2;
         if (common_levels(w, ext, mid, t) >= d) {
@@ -142,32 +180,39 @@ __device__ inline int run_start(const uint64_t* w, const uint64_t* ext, int t, i
 }
 
 // ---------------------------------------------------------------- inputs
-__global__ void round_inputs_kernel(const double* __restrict__ px, const double* __restrict__ py,
+// Build inputs as one (x, y, z, m) record per particle: FP32-rounded and
+// widened back to FP64 (hip_barnes_hut) or the FP64 values (the FP64 walk).
+template <bool ROUND>
+__global__ void build_inputs_kernel(const double* __restrict__ px, const double* __restrict__ py,
                                     const double* __restrict__ pz, const double* __restrict__ m,
-                                    double* __restrict__ bx, double* __restrict__ by,
-                                    double* __restrict__ bz, double* __restrict__ bm, int n) {
+                                    double4* __restrict__ b4, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
-        bx[i] = static_cast<double>(static_cast<float>(px[i]));
-        by[i] = static_cast<double>(static_cast<float>(py[i]));
-        bz[i] = static_cast<double>(static_cast<float>(pz[i]));
-        bm[i] = static_cast<double>(static_cast<float>(m[i]));
+        if (ROUND) {
+            b4[i] = make_double4(static_cast<double>(static_cast<float>(px[i])),
+                                 static_cast<double>(static_cast<float>(py[i])),
+                                 static_cast<double>(static_cast<float>(pz[i])),
+                                 static_cast<double>(static_cast<float>(m[i])));
+        } else {
+            b4[i] = make_double4(px[i], py[i], pz[i], m[i]);
+        }
     }
 }
 
 // ---------------------------------------------------------------- bounding cube
 // barnes_hut.c:453-483. min/max are exact, so the reduction order is free.
-__global__ __launch_bounds__(kThreads) void bbox_kernel(const double* __restrict__ bx,
-                                                       const double* __restrict__ by,
-                                                       const double* __restrict__ bz, int n,
-                                                       double* __restrict__ partial,
-                                                       unsigned* __restrict__ done,
-                                                       double* __restrict__ root) {
+constexpr int kBBoxBlocks = 512;
+
+__global__ __launch_bounds__(kThreads) void bbox_partial_kernel(const double4* __restrict__ b4,
+                                                               int n,
+                                                               double* __restrict__ partial) {
     __shared__ double s[6][kThreads];
-    double lo[3] = {bx[0], by[0], bz[0]};
+    const double4 p0 = b4[0];
+    double lo[3] = {p0.x, p0.y, p0.z};
     double hi[3] = {lo[0], lo[1], lo[2]};
     for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
-        const double p[3] = {bx[i], by[i], bz[i]};
+        const double4 q = b4[i];
+        const double p[3] = {q.x, q.y, q.z};
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             if (p[a] < lo[a]) lo[a] = p[a];
@@ -192,53 +237,62 @@ __global__ __launch_bounds__(kThreads) void bbox_kernel(const double* __restrict
         }
         __syncthreads();
     }
-    __shared__ bool last;
-    if (threadIdx.x == 0) {
-        for (int a = 0; a < 6; ++a) {
-            __hip_atomic_store(&partial[blockIdx.x * 6 + a], s[a][0], __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x < 6) {
+        partial[blockIdx.x * 6 + threadIdx.x] = s[threadIdx.x][0];
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void bbox_final_kernel(const double* __restrict__ partial,
+                                                             int blocks,
+                                                             double* __restrict__ root) {
+    __shared__ double s[6][kThreads];
+    double lo[3] = {partial[0], partial[1], partial[2]};
+    double hi[3] = {partial[3], partial[4], partial[5]};
+    for (int b = threadIdx.x; b < blocks; b += kThreads) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const double l = partial[b * 6 + a], h = partial[b * 6 + 3 + a];
+            if (l < lo[a]) lo[a] = l;
+            if (h > hi[a]) hi[a] = h;
         }
-        __threadfence();
-        const unsigned prev = atomicAdd(done, 1u);
-        last = (prev + 1 == gridDim.x);
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        s[a][threadIdx.x] = lo[a];
+        s[3 + a][threadIdx.x] = hi[a];
     }
     __syncthreads();
-    if (!last || threadIdx.x != 0) {
-        return;
-    }
-    __threadfence();
-    double l[3], h[3];
-    for (int a = 0; a < 3; ++a) {
-        l[a] = __hip_atomic_load(&partial[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        h[a] = __hip_atomic_load(&partial[3 + a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    for (unsigned b = 1; b < gridDim.x; ++b) {
-        for (int a = 0; a < 3; ++a) {
-            const double pl = __hip_atomic_load(&partial[b * 6 + a], __ATOMIC_RELAXED,
-                                                __HIP_MEMORY_SCOPE_AGENT);
-            const double ph = __hip_atomic_load(&partial[b * 6 + 3 + a], __ATOMIC_RELAXED,
-                                                __HIP_MEMORY_SCOPE_AGENT);
-            if (pl < l[a]) l[a] = pl;
-            if (ph > h[a]) h[a] = ph;
+    for (int off = kThreads / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < static_cast<unsigned>(off)) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const double l = s[a][threadIdx.x + off];
+                const double h = s[3 + a][threadIdx.x + off];
+                if (l < s[a][threadIdx.x]) s[a][threadIdx.x] = l;
+                if (h > s[3 + a][threadIdx.x]) s[3 + a][threadIdx.x] = h;
+            }
         }
+        __syncthreads();
     }
-    double half = 0.5 * (h[0] - l[0]);
-    const double hy = 0.5 * (h[1] - l[1]);
-    const double hz = 0.5 * (h[2] - l[2]);
-    if (hy > half) half = hy;
-    if (hz > half) half = hz;
-    if (half == 0.0) half = 0.5;
-    half *= 1.0 + 1.0e-9;
-    root[0] = 0.5 * (l[0] + h[0]);
-    root[1] = 0.5 * (l[1] + h[1]);
-    root[2] = 0.5 * (l[2] + h[2]);
-    root[3] = half;
-    *done = 0;
+    if (threadIdx.x == 0) {
+        const double l[3] = {s[0][0], s[1][0], s[2][0]};
+        const double h[3] = {s[3][0], s[4][0], s[5][0]};
+        double half = 0.5 * (h[0] - l[0]);
+        const double hy = 0.5 * (h[1] - l[1]);
+        const double hz = 0.5 * (h[2] - l[2]);
+        if (hy > half) half = hy;
+        if (hz > half) half = hz;
+        if (half == 0.0) half = 0.5;
+        half *= 1.0 + 1.0e-9;
+        root[0] = 0.5 * (l[0] + h[0]);
+        root[1] = 0.5 * (l[1] + h[1]);
+        root[2] = 0.5 * (l[2] + h[2]);
+        root[3] = half;
+    }
 }
 
 // ---------------------------------------------------------------- keys
-__global__ void keys_kernel(const double* __restrict__ bx, const double* __restrict__ by,
-                            const double* __restrict__ bz, int n, const double* __restrict__ root,
+__global__ void keys_kernel(const double4* __restrict__ b4, int n, const double* __restrict__ root,
                             uint64_t* __restrict__ mkey, uint64_t* __restrict__ tkey,
                             uint64_t* __restrict__ sortkey, uint32_t* __restrict__ idx,
                             unsigned* __restrict__ flags) {
@@ -247,7 +301,8 @@ __global__ void keys_kernel(const double* __restrict__ bx, const double* __restr
         return;
     }
     const double cx = root[0], cy = root[1], cz = root[2], h = root[3];
-    const double x = bx[i], y = by[i], z = bz[i];
+    const double4 p = b4[i];
+    const double x = p.x, y = p.y, z = p.z;
     const uint64_t mk = spread3(axis_code(x, cx, h)) | (spread3(axis_code(y, cy, h)) << 1) |
                         (spread3(axis_code(z, cz, h)) << 2);
     double ccx = cx, ccy = cy, ccz = cz, hh = h;
@@ -262,16 +317,15 @@ __global__ void keys_kernel(const double* __restrict__ bx, const double* __restr
 }
 
 // ext[pos*6+k] = octant digits of levels 21(k+1) .. 21(k+1)+20 of particle orig[pos].
-__global__ void ext_words_kernel(const double* __restrict__ bx, const double* __restrict__ by,
-                                 const double* __restrict__ bz, int n,
+__global__ void ext_words_kernel(const double4* __restrict__ b4, int n,
                                  const double* __restrict__ root,
                                  const uint32_t* __restrict__ orig, uint64_t* __restrict__ ext) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) {
         return;
     }
-    const uint32_t i = orig[p];
-    const double x = bx[i], y = by[i], z = bz[i];
+    const double4 q = b4[orig[p]];
+    const double x = q.x, y = q.y, z = q.z;
     double cx = root[0], cy = root[1], cz = root[2], h = root[3];
     path_word(x, y, z, cx, cy, cz, h);
     for (int k = 0; k < kExtWords; ++k) {
@@ -312,9 +366,7 @@ __global__ void tree_order_kernel(const uint32_t* __restrict__ asc_rank,
                                   const uint32_t* __restrict__ mperm,
                                   const uint64_t* __restrict__ tkey_orig,
                                   const uint64_t* __restrict__ ext_asc,
-                                  const double* __restrict__ bx, const double* __restrict__ by,
-                                  const double* __restrict__ bz, const double* __restrict__ bm,
-                                  int n, uint32_t* __restrict__ tperm, uint32_t* __restrict__ trank,
+                                  const double4* __restrict__ b4, int n, uint32_t* __restrict__ tperm, uint32_t* __restrict__ trank,
                                   uint64_t* __restrict__ tword, uint64_t* __restrict__ ext_tree,
                                   double* __restrict__ sx, double* __restrict__ sy,
                                   double* __restrict__ sz, double* __restrict__ sm,
@@ -335,7 +387,8 @@ __global__ void tree_order_kernel(const uint32_t* __restrict__ asc_rank,
                 ext_asc[static_cast<size_t>(k) * kExtWords + w];
         }
     }
-    const double x = bx[o], y = by[o], z = bz[o], m = bm[o];
+    const double4 p = b4[o];
+    const double x = p.x, y = p.y, z = p.z, m = p.w;
     sx[t] = x;
     sy[t] = y;
     sz[t] = z;
@@ -345,23 +398,56 @@ __global__ void tree_order_kernel(const uint32_t* __restrict__ asc_rank,
 }
 
 // ---------------------------------------------------------------- topology
-__global__ void delta_kernel(const uint64_t* __restrict__ w, const uint64_t* __restrict__ ext,
-                             int n, int32_t* __restrict__ delta, uint32_t* __restrict__ cnt,
-                             unsigned* __restrict__ flags) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) {
-        return;
+// delta[t] = common levels of tree positions t and t+1; cnt[t] = internal
+// nodes started at t (depths delta[t-1]+1 .. delta[t]); dcount[d] = internal
+// nodes at depth d (block-aggregated in LDS before one global add per depth).
+__global__ __launch_bounds__(kThreads) void delta_kernel(const uint64_t* __restrict__ w,
+                                                        const uint64_t* __restrict__ ext, int n,
+                                                        int32_t* __restrict__ delta,
+                                                        uint32_t* __restrict__ cnt,
+                                                        unsigned* __restrict__ flags,
+                                                        unsigned* __restrict__ dcount) {
+    __shared__ unsigned hist[kMaxLevels];
+    for (int d = threadIdx.x; d < kMaxLevels; d += kThreads) {
+        hist[d] = 0;
     }
-    const int dprev = t > 0 ? common_levels(w, ext, t - 1, t) : -1;
-    const int dcur = t < n - 1 ? common_levels(w, ext, t, t + 1) : -1;
-    delta[t] = dcur;
-    const int c = t == 0 ? dcur + 1 : (dcur > dprev ? dcur - dprev : 0);
-    cnt[t] = static_cast<uint32_t>(c > 0 ? c : 0);
-    if (ext == nullptr && dcur >= 21) {
-        atomicOr(&flags[FLAG_TIE], 1u);
+    __syncthreads();
+    const int t = blockIdx.x * kThreads + threadIdx.x;
+    if (t < n) {
+        const int dprev = t > 0 ? common_levels(w, ext, t - 1, t) : -1;
+        const int dcur = t < n - 1 ? common_levels(w, ext, t, t + 1) : -1;
+        delta[t] = dcur;
+        const int lo = dprev + 1;
+        const int c = dcur >= lo ? dcur - lo + 1 : 0;
+        cnt[t] = static_cast<uint32_t>(c);
+        const int top = dcur < kMaxLevels ? dcur : kMaxLevels - 1;
+        for (int d = lo; d <= top; ++d) {
+            atomicAdd(&hist[d], 1u);
+        }
+        if (ext == nullptr && dcur >= 21) {
+            atomicOr(&flags[FLAG_TIE], 1u);
+        }
+        if (dcur >= kMaxLevels) {
+            atomicOr(&flags[FLAG_OVERFLOW], 1u);
+        }
     }
-    if (dcur >= kMaxLevels) {
-        atomicOr(&flags[FLAG_OVERFLOW], 1u);
+    __syncthreads();
+    for (int d = threadIdx.x; d < kMaxLevels; d += kThreads) {
+        if (hist[d] != 0) {
+            atomicAdd(&dcount[d], hist[d]);
+        }
+    }
+}
+
+// Exclusive offsets of the per-depth internal-node lists.
+__global__ void depth_offsets_kernel(const unsigned* __restrict__ dcount,
+                                     unsigned* __restrict__ doff) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        unsigned run = 0;
+        for (int d = 0; d < kMaxLevels; ++d) {
+            doff[d] = run;
+            run += dcount[d];
+        }
     }
 }
 
@@ -371,10 +457,6 @@ struct NodeSoA {
     double* cz;
     double* m;
     int32_t* skip;
-    int32_t* leaf;
-    int32_t* parent;
-    uint32_t* nchild;
-    uint32_t* arrive;
     int32_t* depth;
     int32_t* start;
     int32_t* count;
@@ -393,178 +475,230 @@ __device__ inline int leaf_index(const uint32_t* scan, const uint32_t* cnt, int 
     return static_cast<int>(scan[t] + cnt[t]) + t;
 }
 
-__global__ void fill_kernel(const uint64_t* __restrict__ w, const uint64_t* __restrict__ ext,
-                            const int32_t* __restrict__ delta, const uint32_t* __restrict__ cnt,
-                            const uint32_t* __restrict__ scan, int n,
-                            const double* __restrict__ sx, const double* __restrict__ sy,
-                            const double* __restrict__ sz, const double* __restrict__ sm,
-                            NodeSoA nd, unsigned* __restrict__ flags) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) {
+__device__ inline int internal_id(const uint32_t* scan, const int32_t* delta, int s, int d) {
+    return static_cast<int>(scan[s]) + (d - lo_of(delta, s));
+}
+
+// Internal id of the depth-d cell that contains tree position t (d >= 0).
+__device__ inline int cell_id(const uint64_t* w, const uint64_t* ext, const uint32_t* scan,
+                              const int32_t* delta, int t, int d) {
+    const int s = d >= lo_of(delta, t) ? t : run_start(w, ext, t, d);
+    return internal_id(scan, delta, s, d);
+}
+
+// Static node fields, leaf records, the per-depth lists of internal nodes
+// (by internal id), and every node's slot in its parent's child table
+// kids[parent_internal_id * 8 + octant].
+__global__ __launch_bounds__(kThreads) void fill_kernel(
+    const uint64_t* __restrict__ w, const uint64_t* __restrict__ ext,
+    const int32_t* __restrict__ delta, const uint32_t* __restrict__ cnt,
+    const uint32_t* __restrict__ scan, int n, const double* __restrict__ sx,
+    const double* __restrict__ sy, const double* __restrict__ sz, const double* __restrict__ sm,
+    double root_h, NodeSoA nd, NodeF64* __restrict__ n64, NodeF32* __restrict__ n32,
+    const unsigned* __restrict__ doff, unsigned* __restrict__ dcursor, int* __restrict__ dlist,
+    int* __restrict__ inode, int* __restrict__ kids) {
+    __shared__ unsigned local[kMaxLevels];
+    __shared__ unsigned base[kMaxLevels];
+    for (int d = threadIdx.x; d < kMaxLevels; d += kThreads) {
+        local[d] = 0;
+    }
+    __syncthreads();
+    const int t = blockIdx.x * kThreads + threadIdx.x;
+    const bool valid = t < n;
+    const int lo = valid ? lo_of(delta, t) : 0;
+    const int hi = valid ? delta[t] : -1;
+    for (int d = lo; d <= hi; ++d) {
+        atomicAdd(&local[d], 1u);
+    }
+    __syncthreads();
+    for (int d = threadIdx.x; d < kMaxLevels; d += kThreads) {
+        base[d] = local[d] != 0 ? atomicAdd(&dcursor[d], local[d]) : 0u;
+        local[d] = 0;
+    }
+    __syncthreads();
+    if (!valid) {
         return;
     }
-    const int lo = lo_of(delta, t);
-    const int hi = delta[t];
     for (int d = lo; d <= hi; ++d) {
         const int idx = internal_index(scan, delta, t, d);
+        const int iid = internal_id(scan, delta, t, d);
         const int e = run_end(w, ext, n, t, d);
         nd.skip[idx] = leaf_index(scan, cnt, e) + 1;
-        nd.leaf[idx] = -1;
         nd.depth[idx] = d;
         nd.start[idx] = t;
         nd.count[idx] = e - t + 1;
-        int parent = -1;
+        inode[iid] = idx;
+        const unsigned slot = base[d] + atomicAdd(&local[d], 1u);
+        dlist[doff[d] + slot] = iid;
         if (d > 0) {
-            parent = (d - 1 >= lo) ? idx - 1 : internal_index(scan, delta, run_start(w, ext, t, d - 1), d - 1);
-        }
-        nd.parent[idx] = parent;
-        if (parent >= 0) {
-            atomicAdd(&nd.nchild[parent], 1u);
+            const int parent = d - 1 >= lo ? iid - 1 : cell_id(w, ext, scan, delta, t, d - 1);
+            kids[parent * 8 + digit_at(w, ext, t, d - 1)] = idx;
         }
     }
     const int dprev = t > 0 ? delta[t - 1] : -1;
     const int depth = 1 + (dprev > hi ? dprev : hi);
     const int li = leaf_index(scan, cnt, t);
+    if (depth > 0) {
+        const int parent = depth - 1 == hi && hi >= lo ? internal_id(scan, delta, t, hi)
+                                                       : cell_id(w, ext, scan, delta, t, depth - 1);
+        kids[parent * 8 + digit_at(w, ext, t, depth - 1)] = li;
+    }
+    const double x = sx[t], y = sy[t], z = sz[t], m = sm[t];
     nd.skip[li] = li + 1;
-    nd.leaf[li] = t;
     nd.depth[li] = depth;
     nd.start[li] = t;
     nd.count[li] = 1;
-    nd.cx[li] = sx[t];
-    nd.cy[li] = sy[t];
-    nd.cz[li] = sz[t];
-    nd.m[li] = sm[t];
+    nd.cx[li] = x;
+    nd.cy[li] = y;
+    nd.cz[li] = z;
+    nd.m[li] = m;
     nd.minpos[li] = t;
-    int parent = -1;
-    if (n > 1) {
-        const int pd = depth - 1;
-        parent = (pd >= lo && pd <= hi) ? li - 1 : internal_index(scan, delta, run_start(w, ext, t, pd), pd);
-    }
-    nd.parent[li] = parent;
-    if (parent >= 0) {
-        atomicAdd(&nd.nchild[parent], 1u);
-    }
-    atomicMax(&flags[FLAG_MAXDEPTH], static_cast<unsigned>(depth));
-}
-
-__device__ inline double ldc(const double* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ inline int32_t ldc(const int32_t* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ inline void stc(double* p, double v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ inline void stc(int32_t* p, int32_t v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// barnes_hut.c:214-253 (children summed in octant order 0..7; a zero-mass
-// cell keeps the position of the particle that created it: the lowest Morton
-// rank in the cell, or for the root the caller's particle 0, :504-512).
-__global__ void com_kernel(const uint32_t* __restrict__ scan, const uint32_t* __restrict__ cnt,
-                           const uint32_t* __restrict__ trank, int n, NodeSoA nd,
-                           const double* __restrict__ sx, const double* __restrict__ sy,
-                           const double* __restrict__ sz, const double* __restrict__ bx,
-                           const double* __restrict__ by, const double* __restrict__ bz) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) {
-        return;
-    }
-    int node = leaf_index(scan, cnt, t);
-    int p = nd.parent[node];
-    while (p >= 0) {
-        __threadfence();
-        const unsigned prev = atomicAdd(&nd.arrive[p], 1u);
-        if (prev + 1 != nd.nchild[p]) {
-            return;
-        }
-        __threadfence();
-        int kids[8];
-        int k = 0;
-        const int end = nd.skip[p];
-        for (int c = p + 1; c < end && k < 8; c = nd.skip[c]) {
-            kids[k++] = c;
-        }
-        double mass = 0.0, x = 0.0, y = 0.0, z = 0.0;
-        int minpos = -1;
-        uint32_t minrank = 0xFFFFFFFFu;
-        for (int q = k - 1; q >= 0; --q) {
-            const int c = kids[q];
-            const double cm = ldc(&nd.m[c]);
-            mass += cm;
-            x += cm * ldc(&nd.cx[c]);
-            y += cm * ldc(&nd.cy[c]);
-            z += cm * ldc(&nd.cz[c]);
-            const int mp = ldc(&nd.minpos[c]);
-            const uint32_t r = trank[mp];
-            if (r < minrank) {
-                minrank = r;
-                minpos = mp;
-            }
-        }
-        double ox, oy, oz;
-        if (mass != 0.0) {
-            ox = x / mass;
-            oy = y / mass;
-            oz = z / mass;
-        } else if (p == 0) {
-            ox = bx[0];
-            oy = by[0];
-            oz = bz[0];
-        } else {
-            ox = sx[minpos];
-            oy = sy[minpos];
-            oz = sz[minpos];
-        }
-        stc(&nd.m[p], mass);
-        stc(&nd.cx[p], ox);
-        stc(&nd.cy[p], oy);
-        stc(&nd.cz[p], oz);
-        stc(&nd.minpos[p], minpos);
-        node = p;
-        p = nd.parent[node];
-    }
-}
-
-__global__ void pack_kernel(NodeSoA nd, int nodes, double root_h, NodeF64* __restrict__ n64,
-                            NodeF32* __restrict__ n32) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nodes) {
-        return;
-    }
-    const double h = ldexp(root_h, -nd.depth[i]);
+    const double h = ldexp(root_h, -depth);
     NodeF64 a;
-    a.cx = nd.cx[i];
-    a.cy = nd.cy[i];
-    a.cz = nd.cz[i];
-    a.m = nd.m[i];
+    a.cx = x;
+    a.cy = y;
+    a.cz = z;
+    a.m = m;
     a.h = h;
-    a.skip = nd.skip[i];
-    a.leaf = nd.leaf[i];
-    n64[i] = a;
+    a.skip = li + 1;
+    a.leaf = t;
+    n64[li] = a;
     NodeF32 b;
-    b.cx = static_cast<float>(a.cx);
-    b.cy = static_cast<float>(a.cy);
-    b.cz = static_cast<float>(a.cz);
-    b.m = static_cast<float>(a.m);
-    b.size2 = static_cast<float>(4.0 * h * h);
-    b.skip = a.skip;
-    b.leaf = a.leaf;
-    b.depth = nd.depth[i];
-    n32[i] = b;
+    b.cx = static_cast<float>(x);
+    b.cy = static_cast<float>(y);
+    b.cz = static_cast<float>(z);
+    b.m = static_cast<float>(m);
+    b.size2 = 0.f;
+    b.skip = li + 1;
+    b.leaf = t;
+    b.depth = depth;
+    n32[li] = b;
 }
+
+// One node of the centre-of-mass pass (barnes_hut.c:214-253): children
+// summed in octant order 0..7; a zero-mass cell keeps the position of the
+// particle that created it (lowest Morton rank in the cell; for the root the
+// caller's particle 0, barnes_hut.c:504-512). Children are one level deeper
+// and were finished by the previous launch.
+__device__ inline void com_node(int iid, const int* __restrict__ inode,
+                                const int* __restrict__ kids, NodeSoA nd,
+                                const uint32_t* __restrict__ trank,
+                                const double* __restrict__ sx, const double* __restrict__ sy,
+                                const double* __restrict__ sz, const double4* __restrict__ b4,
+                                double root_h, NodeF64* __restrict__ n64,
+                                NodeF32* __restrict__ n32) {
+    const int p = inode[iid];
+    const int end = nd.skip[p];
+    int kid[8];
+#pragma unroll
+    for (int o = 0; o < 8; ++o) {
+        kid[o] = kids[iid * 8 + o];
+    }
+    double mass = 0.0, x = 0.0, y = 0.0, z = 0.0;
+    int minpos = -1;
+    uint32_t minrank = 0xFFFFFFFFu;
+#pragma unroll
+    for (int o = 0; o < 8; ++o) {
+        const int c = kid[o];
+        if (c < 0) {
+            continue;
+        }
+        const double cm = nd.m[c];
+        mass += cm;
+        x += cm * nd.cx[c];
+        y += cm * nd.cy[c];
+        z += cm * nd.cz[c];
+        const int mp = nd.minpos[c];
+        const uint32_t r = trank[mp];
+        if (r < minrank) {
+            minrank = r;
+            minpos = mp;
+        }
+    }
+    double ox, oy, oz;
+    if (mass != 0.0) {
+        ox = x / mass;
+        oy = y / mass;
+        oz = z / mass;
+    } else if (p == 0) {
+        const double4 p0 = b4[0];
+        ox = p0.x;
+        oy = p0.y;
+        oz = p0.z;
+    } else {
+        ox = sx[minpos];
+        oy = sy[minpos];
+        oz = sz[minpos];
+    }
+    nd.m[p] = mass;
+    nd.cx[p] = ox;
+    nd.cy[p] = oy;
+    nd.cz[p] = oz;
+    nd.minpos[p] = minpos;
+    const int depth = nd.depth[p];
+    const double h = ldexp(root_h, -depth);
+    NodeF64 a;
+    a.cx = ox;
+    a.cy = oy;
+    a.cz = oz;
+    a.m = mass;
+    a.h = h;
+    a.skip = end;
+    a.leaf = -1;
+    n64[p] = a;
+    NodeF32 b;
+    b.cx = static_cast<float>(ox);
+    b.cy = static_cast<float>(oy);
+    b.cz = static_cast<float>(oz);
+    b.m = static_cast<float>(mass);
+    b.size2 = static_cast<float>(4.0 * h * h);
+    b.skip = end;
+    b.leaf = -1;
+    b.depth = depth;
+    n32[p] = b;
+}
+
+// One depth level per launch (large levels).
+__global__ __launch_bounds__(kThreads) void com_level_kernel(
+    const int* __restrict__ list, int count, const int* __restrict__ inode,
+    const int* __restrict__ kids, NodeSoA nd, const uint32_t* __restrict__ trank,
+    const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
+    const double4* __restrict__ b4, double root_h, NodeF64* __restrict__ n64,
+    NodeF32* __restrict__ n32) {
+    const int k = blockIdx.x * kThreads + threadIdx.x;
+    if (k < count) {
+        com_node(list[k], inode, kids, nd, trank, sx, sy, sz, b4, root_h, n64, n32);
+    }
+}
+
+// Consecutive small depth levels (deepest first) in one workgroup; the
+// barrier orders each level's results before the next level reads them.
+__global__ __launch_bounds__(1024) void com_small_levels_kernel(
+    const int* __restrict__ list, const unsigned* __restrict__ doff,
+    const unsigned* __restrict__ dcount, int d_hi, int d_lo, const int* __restrict__ inode,
+    const int* __restrict__ kids, NodeSoA nd,
+    const uint32_t* __restrict__ trank, const double* __restrict__ sx,
+    const double* __restrict__ sy, const double* __restrict__ sz, const double4* __restrict__ b4,
+    double root_h, NodeF64* __restrict__ n64, NodeF32* __restrict__ n32) {
+    for (int d = d_hi; d >= d_lo; --d) {
+        const int count = static_cast<int>(dcount[d]);
+        const int* lst = list + doff[d];
+        for (int k = threadIdx.x; k < count; k += blockDim.x) {
+            com_node(lst[k], inode, kids, nd, trank, sx, sy, sz, b4, root_h, n64, n32);
+        }
+        __syncthreads();
+    }
+}
+
 
 // ---------------------------------------------------------------- walk
-template <bool FP64>
-__global__ __launch_bounds__(kThreads) void walk_kernel(
-    const NodeF64* __restrict__ n64, const NodeF32* __restrict__ n32, int nodes,
-    const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
-    const float4* __restrict__ s4, const uint32_t* __restrict__ tperm, int n, double G,
-    double eps2, double theta2, double* __restrict__ ax, double* __restrict__ ay,
-    double* __restrict__ az, unsigned long long* __restrict__ counters,
-    float* __restrict__ features) {
+// FP64 variant: the reference arithmetic (barnes_hut.c:574-613), bit-identical.
+__global__ __launch_bounds__(kThreads) void walk_f64_kernel(
+    const NodeF64* __restrict__ n64, int nodes, const double* __restrict__ sx,
+    const double* __restrict__ sy, const double* __restrict__ sz,
+    const uint32_t* __restrict__ tperm, int n, double G, double eps2, double theta2,
+    double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az,
+    unsigned long long* __restrict__ counters) {
     __shared__ unsigned long long block_counts[2];
     if (threadIdx.x < 2) {
         block_counts[threadIdx.x] = 0;
@@ -575,131 +709,160 @@ __global__ __launch_bounds__(kThreads) void walk_kernel(
     const int tc = valid ? t : n - 1;
     int skip_until = valid ? 0 : 0x7FFFFFFF;
     unsigned approx = 0, exact = 0;
-    float ratio_sum = 0.f;
-
-    if (FP64) {
-        const double qx = sx[tc], qy = sy[tc], qz = sz[tc];
-        double acx = 0.0, acy = 0.0, acz = 0.0;
-        int i = 0;
-        while (i < nodes) {
-            const NodeF64 nd = n64[__builtin_amdgcn_readfirstlane(i)];
-            const bool active = i >= skip_until;
-            if (nd.leaf >= 0) {
-                if (active && nd.leaf != t) {
-                    const double dx = nd.cx - qx;
-                    const double dy = nd.cy - qy;
-                    const double dz = nd.cz - qz;
-                    const double r2 = dx * dx + dy * dy + dz * dz + eps2;
+    const double qx = sx[tc], qy = sy[tc], qz = sz[tc];
+    double acx = 0.0, acy = 0.0, acz = 0.0;
+    int i = 0;
+    while (i < nodes) {
+        const NodeF64 nd = n64[__builtin_amdgcn_readfirstlane(i)];
+        const bool active = i >= skip_until;
+        if (nd.leaf >= 0) {
+            if (active && nd.leaf != t) {
+                const double dx = nd.cx - qx;
+                const double dy = nd.cy - qy;
+                const double dz = nd.cz - qz;
+                const double r2 = dx * dx + dy * dy + dz * dz + eps2;
+                const double inv3 = 1.0 / (r2 * sqrt(r2));
+                const double s = G * nd.m * inv3;
+                acx += s * dx;
+                acy += s * dy;
+                acz += s * dz;
+                ++exact;
+            }
+            i = i + 1;
+        } else {
+            bool descend = false;
+            if (active) {
+                const double dx = nd.cx - qx;
+                const double dy = nd.cy - qy;
+                const double dz = nd.cz - qz;
+                const double d2 = dx * dx + dy * dy + dz * dz;
+                const double size2 = 4.0 * nd.h * nd.h;
+                if (d2 > 0.0 && size2 < theta2 * d2) {
+                    const double r2 = d2 + eps2;
                     const double inv3 = 1.0 / (r2 * sqrt(r2));
                     const double s = G * nd.m * inv3;
                     acx += s * dx;
                     acy += s * dy;
                     acz += s * dz;
-                    ++exact;
+                    ++approx;
+                    skip_until = nd.skip;
+                } else {
+                    descend = true;
                 }
-                i = i + 1;
-            } else {
-                bool descend = false;
-                if (active) {
-                    const double dx = nd.cx - qx;
-                    const double dy = nd.cy - qy;
-                    const double dz = nd.cz - qz;
-                    const double d2 = dx * dx + dy * dy + dz * dz;
-                    const double size2 = 4.0 * nd.h * nd.h;
-                    if (d2 > 0.0 && size2 < theta2 * d2) {
-                        const double r2 = d2 + eps2;
-                        const double inv3 = 1.0 / (r2 * sqrt(r2));
-                        const double s = G * nd.m * inv3;
-                        acx += s * dx;
-                        acy += s * dy;
-                        acz += s * dz;
-                        ++approx;
-                        skip_until = nd.skip;
-                    } else {
-                        descend = true;
-                    }
-                }
-                i = __ballot(descend) != 0 ? i + 1 : nd.skip;
             }
-        }
-        if (valid) {
-            const uint32_t o = tperm[t];
-            ax[o] = acx;
-            ay[o] = acy;
-            az[o] = acz;
-        }
-    } else {
-        const float4 q = s4[tc];
-        const float Gf = static_cast<float>(G);
-        const float e2 = static_cast<float>(eps2);
-        const float th2 = static_cast<float>(theta2);
-        float acx = 0.f, acy = 0.f, acz = 0.f;
-        int i = 0;
-        while (i < nodes) {
-            const NodeF32 nd = n32[__builtin_amdgcn_readfirstlane(i)];
-            const bool active = i >= skip_until;
-            if (nd.leaf >= 0) {
-                if (active && nd.leaf != t) {
-                    const float dx = nd.cx - q.x;
-                    const float dy = nd.cy - q.y;
-                    const float dz = nd.cz - q.z;
-                    const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, e2)));
-                    const float inv = __builtin_amdgcn_rsqf(r2);
-                    const float s = (Gf * nd.m * inv) * (inv * inv);
-                    acx = fmaf(s, dx, acx);
-                    acy = fmaf(s, dy, acy);
-                    acz = fmaf(s, dz, acz);
-                    ++exact;
-                }
-                i = i + 1;
-            } else {
-                bool descend = false;
-                if (active) {
-                    const float dx = nd.cx - q.x;
-                    const float dy = nd.cy - q.y;
-                    const float dz = nd.cz - q.z;
-                    const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                    if (d2 > 0.f && nd.size2 < th2 * d2) {
-                        const float r2 = d2 + e2;
-                        const float inv = __builtin_amdgcn_rsqf(r2);
-                        const float s = (Gf * nd.m * inv) * (inv * inv);
-                        acx = fmaf(s, dx, acx);
-                        acy = fmaf(s, dy, acy);
-                        acz = fmaf(s, dz, acz);
-                        ++approx;
-                        if (features != nullptr) {
-                            ratio_sum += sqrtf(d2 / nd.size2);
-                        }
-                        skip_until = nd.skip;
-                    } else {
-                        descend = true;
-                    }
-                }
-                i = __ballot(descend) != 0 ? i + 1 : nd.skip;
-            }
-        }
-        if (valid) {
-            const uint32_t o = tperm[t];
-            ax[o] = acx;
-            ay[o] = acy;
-            az[o] = acz;
-            if (features != nullptr) {
-                // per tree position: accepted cells, exact leaves, mean accepted d/size
-                features[static_cast<size_t>(t) * 4 + 0] = static_cast<float>(approx);
-                features[static_cast<size_t>(t) * 4 + 1] = static_cast<float>(exact);
-                features[static_cast<size_t>(t) * 4 + 2] =
-                    approx > 0 ? ratio_sum / static_cast<float>(approx) : 0.f;
-                features[static_cast<size_t>(t) * 4 + 3] = static_cast<float>(o);
-            }
+            i = __ballot(descend) != 0 ? i + 1 : nd.skip;
         }
     }
     if (valid) {
+        const uint32_t o = tperm[t];
+        ax[o] = acx;
+        ay[o] = acy;
+        az[o] = acz;
         atomicAdd(&block_counts[0], static_cast<unsigned long long>(approx));
         atomicAdd(&block_counts[1], static_cast<unsigned long long>(exact));
     }
     __syncthreads();
     if (threadIdx.x < 2) {
         atomicAdd(&counters[threadIdx.x], block_counts[threadIdx.x]);
+    }
+}
+
+// MI355X dispatches workgroup b to XCD (b mod 8), and each XCD has its own L2.
+// Remap so that XCD x walks one contiguous eighth of the queries (Morton
+// order): its L2 then holds that eighth's part of the tree. `grid` is a
+// multiple of kXcds; surplus blocks get indices >= n and only walk the root.
+constexpr int kXcds = 8;
+__device__ inline int xcd_block(int b, int grid) {
+    return (b % kXcds) * (grid / kXcds) + b / kXcds;
+}
+
+// FP32 variant: one node per wave-step, node fields in SGPRs (scalar loads of
+// a wave-uniform index). Lanes with nothing to do at a node are masked off,
+// and when no lane is active the arithmetic is skipped outright. G is
+// applied once at the end. FEATURES additionally records per-query walk
+// statistics for the learned correction. (Measured alternatives, slower on
+// MI355X: branch-free steps, vector-loaded nodes, prefetching both
+// successors; the walk is bound by node fetch latency, not VALU issue.)
+template <bool FEATURES>
+__global__ __launch_bounds__(kThreads) void walk_f32_kernel(
+    const NodeF32* __restrict__ n32, int nodes, const float4* __restrict__ s4,
+    const uint32_t* __restrict__ tperm, int n, float G, float eps2, float theta2,
+    double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az,
+    unsigned long long* __restrict__ counters, float* __restrict__ features) {
+    const int t = xcd_block(blockIdx.x, gridDim.x) * kThreads + threadIdx.x;
+    const bool valid = t < n;
+    const float4 q = s4[valid ? t : n - 1];
+    int skip_until = valid ? 0 : 0x7FFFFFFF;
+    unsigned approx = 0, exact = 0;
+    float ratio_sum = 0.f;
+    float acx = 0.f, acy = 0.f, acz = 0.f;
+    int i = 0;
+    while (i < nodes) {
+        const NodeF32 cur = n32[__builtin_amdgcn_readfirstlane(i)];
+        const bool active = i >= skip_until;
+        if (cur.leaf >= 0) {
+            if (active && cur.leaf != t) {
+                const float dx = cur.cx - q.x;
+                const float dy = cur.cy - q.y;
+                const float dz = cur.cz - q.z;
+                const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
+                const float inv = __builtin_amdgcn_rsqf(r2);
+                const float s = (cur.m * inv) * (inv * inv);
+                acx = fmaf(s, dx, acx);
+                acy = fmaf(s, dy, acy);
+                acz = fmaf(s, dz, acz);
+                ++exact;
+            }
+            i = i + 1;
+        } else {
+            bool descend = false;
+            if (active) {
+                const float dx = cur.cx - q.x;
+                const float dy = cur.cy - q.y;
+                const float dz = cur.cz - q.z;
+                const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                if (d2 > 0.f && cur.size2 < theta2 * d2) {
+                    const float r2 = d2 + eps2;
+                    const float inv = __builtin_amdgcn_rsqf(r2);
+                    const float s = (cur.m * inv) * (inv * inv);
+                    acx = fmaf(s, dx, acx);
+                    acy = fmaf(s, dy, acy);
+                    acz = fmaf(s, dz, acz);
+                    ++approx;
+                    if (FEATURES) {
+                        ratio_sum += __builtin_amdgcn_sqrtf(d2) * __builtin_amdgcn_rsqf(cur.size2);
+                    }
+                    skip_until = cur.skip;
+                } else {
+                    descend = true;
+                }
+            }
+            i = __builtin_amdgcn_ballot_w64(descend) != 0 ? i + 1 : cur.skip;
+        }
+    }
+    if (valid) {
+        const uint32_t o = tperm[t];
+        ax[o] = static_cast<double>(G * acx);
+        ay[o] = static_cast<double>(G * acy);
+        az[o] = static_cast<double>(G * acz);
+        if (FEATURES) {
+            // per tree position: accepted cells, exact leaves, mean accepted distance/size
+            features[static_cast<size_t>(t) * 4 + 0] = static_cast<float>(approx);
+            features[static_cast<size_t>(t) * 4 + 1] = static_cast<float>(exact);
+            features[static_cast<size_t>(t) * 4 + 2] =
+                approx > 0 ? ratio_sum / static_cast<float>(approx) : 0.f;
+            features[static_cast<size_t>(t) * 4 + 3] = 0.f;
+        }
+    }
+    // wave totals: one atomic per wave
+    unsigned long long wa = valid ? approx : 0u, we = valid ? exact : 0u;
+    for (int off = 32; off > 0; off >>= 1) {
+        wa += __shfl_xor(wa, off);
+        we += __shfl_xor(we, off);
+    }
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        atomicAdd(&counters[0], wa);
+        atomicAdd(&counters[1], we);
     }
 }
 
@@ -730,16 +893,19 @@ NodeSoA node_soa(BHState& s) {
     nd.cz = B<double>(s.ncz);
     nd.m = B<double>(s.nm);
     nd.skip = B<int32_t>(s.nskip);
-    nd.leaf = B<int32_t>(s.nleaf);
-    nd.parent = B<int32_t>(s.nparent);
-    nd.nchild = B<uint32_t>(s.nchild);
-    nd.arrive = B<uint32_t>(s.narrive);
     nd.depth = B<int32_t>(s.ndepth);
     nd.start = B<int32_t>(s.nstart);
     nd.count = B<int32_t>(s.ncount);
     nd.minpos = B<int32_t>(s.nminpos);
     return nd;
 }
+
+// Device word layout of s.flags: [0..7] flags, [8..8+129) depth counts,
+// [256..) depth offsets, [512..) depth cursors.
+constexpr int kDCount = 8;
+constexpr int kDOff = 256;
+constexpr int kDCursor = 512;
+constexpr int kFlagWords = 768;
 
 }  // namespace
 
@@ -758,24 +924,16 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error)
     const unsigned g = grid_for(n, kThreads);
 
     NBX_HIP(error, hipEventRecord(e->ev[0], st));
-    const double *bx = e->px, *by = e->py, *bz = e->pz, *bm = e->m;
+    NBX_TRY(s.b4.need<double4>(n, error));
+    double4* b4 = B<double4>(s.b4);
     if (fp32_inputs) {
-        NBX_TRY(s.bx.need<double>(n, error));
-        NBX_TRY(s.by.need<double>(n, error));
-        NBX_TRY(s.bz.need<double>(n, error));
-        NBX_TRY(s.bm.need<double>(n, error));
-        round_inputs_kernel<<<g, kThreads, 0, st>>>(e->px, e->py, e->pz, e->m, B<double>(s.bx),
-                                                    B<double>(s.by), B<double>(s.bz),
-                                                    B<double>(s.bm), ni);
-        bx = B<double>(s.bx);
-        by = B<double>(s.by);
-        bz = B<double>(s.bz);
-        bm = B<double>(s.bm);
+        build_inputs_kernel<true><<<g, kThreads, 0, st>>>(e->px, e->py, e->pz, e->m, b4, ni);
+    } else {
+        build_inputs_kernel<false><<<g, kThreads, 0, st>>>(e->px, e->py, e->pz, e->m, b4, ni);
     }
-    const unsigned bb_blocks = g < 1024u ? g : 1024u;
-    NBX_TRY(s.partial.need<double>(6 * 1024 + 8, error));
-    NBX_TRY(s.counter.need<unsigned>(4, error));
-    NBX_TRY(s.flags.need<unsigned>(FLAG_COUNT, error));
+    const unsigned bb_blocks = g < static_cast<unsigned>(kBBoxBlocks) ? g : kBBoxBlocks;
+    NBX_TRY(s.partial.need<double>(6 * kBBoxBlocks + 8, error));
+    NBX_TRY(s.flags.need<unsigned>(kFlagWords, error));
     NBX_TRY(s.mkey.need<uint64_t>(n, error));
     NBX_TRY(s.tkey.need<uint64_t>(n, error));
     NBX_TRY(s.sk0.need<uint64_t>(n, error));
@@ -798,13 +956,13 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error)
     NBX_TRY(s.s4.need<float4>(n, error));
     NBX_TRY(s.counters.need<unsigned long long>(2, error));
 
-    double* root = B<double>(s.partial) + 6 * 1024;
+    double* root = B<double>(s.partial) + 6 * kBBoxBlocks;
     unsigned* flags = B<unsigned>(s.flags);
-    NBX_HIP(error, hipMemsetAsync(flags, 0, FLAG_COUNT * sizeof(unsigned), st));
-    NBX_HIP(error, hipMemsetAsync(s.counter.p, 0, 4 * sizeof(unsigned), st));
-    bbox_kernel<<<bb_blocks, kThreads, 0, st>>>(bx, by, bz, ni, B<double>(s.partial),
-                                                B<unsigned>(s.counter), root);
-    keys_kernel<<<g, kThreads, 0, st>>>(bx, by, bz, ni, root, B<uint64_t>(s.mkey),
+    NBX_HIP(error, hipMemsetAsync(flags, 0, kFlagWords * sizeof(unsigned), st));
+    bbox_partial_kernel<<<bb_blocks, kThreads, 0, st>>>(b4, ni, B<double>(s.partial));
+    bbox_final_kernel<<<1, kThreads, 0, st>>>(B<double>(s.partial), static_cast<int>(bb_blocks),
+                                              root);
+    keys_kernel<<<g, kThreads, 0, st>>>(b4, ni, root, B<uint64_t>(s.mkey),
                                         B<uint64_t>(s.tkey), B<uint64_t>(s.sk0),
                                         B<uint32_t>(s.sv0), flags);
     NBX_HIP(error, hipGetLastError());
@@ -817,8 +975,8 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error)
                                   hipMemcpyDeviceToDevice, st));
     NBX_HIP(error, hipMemcpyAsync(s.mperm.p, s.sv0.p, n * sizeof(uint32_t),
                                   hipMemcpyDeviceToDevice, st));
-    unsigned hflags[FLAG_COUNT];
-    NBX_HIP(error, hipMemcpyAsync(hflags, flags, sizeof(hflags), hipMemcpyDeviceToHost, st));
+    unsigned hflags[kDOff];
+    NBX_HIP(error, hipMemcpyAsync(hflags, flags, 8 * sizeof(unsigned), hipMemcpyDeviceToHost, st));
     NBX_HIP(error, hipMemcpyAsync(s.root, root, 4 * sizeof(double), hipMemcpyDeviceToHost, st));
     NBX_HIP(error, hipStreamSynchronize(st));
     s.mismatch_path = hflags[FLAG_MISMATCH] != 0;
@@ -836,13 +994,14 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error)
         asc_rank = B<uint32_t>(s.tmp32);
     }
     tree_order_kernel<<<g, kThreads, 0, st>>>(
-        asc_rank, B<uint32_t>(s.mperm), B<uint64_t>(s.tkey), nullptr, bx, by, bz, bm, ni,
+        asc_rank, B<uint32_t>(s.mperm), B<uint64_t>(s.tkey), nullptr, b4, ni,
         B<uint32_t>(s.tperm), B<uint32_t>(s.trank), B<uint64_t>(s.tword), nullptr, B<double>(s.sx),
         B<double>(s.sy), B<double>(s.sz), B<double>(s.sm), B<float4>(s.s4));
     delta_kernel<<<g, kThreads, 0, st>>>(B<uint64_t>(s.tword), nullptr, ni, B<int32_t>(s.delta),
-                                         B<uint32_t>(s.cnt), flags);
+                                         B<uint32_t>(s.cnt), flags, flags + kDCount);
     NBX_HIP(error, hipGetLastError());
-    NBX_HIP(error, hipMemcpyAsync(hflags, flags, sizeof(hflags), hipMemcpyDeviceToHost, st));
+    NBX_HIP(error, hipMemcpyAsync(hflags, flags, kDOff * sizeof(unsigned), hipMemcpyDeviceToHost,
+                                  st));
     NBX_HIP(error, hipStreamSynchronize(st));
 
     const uint64_t* ext_tree = nullptr;
@@ -855,7 +1014,7 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error)
         uint64_t* ext_m = B<uint64_t>(s.ext);
         uint64_t* ext_a = ext_m + n * kExtWords;
         uint64_t* ext_t = ext_a + n * kExtWords;
-        ext_words_kernel<<<g, kThreads, 0, st>>>(bx, by, bz, ni, root, B<uint32_t>(s.mperm), ext_m);
+        ext_words_kernel<<<g, kThreads, 0, st>>>(b4, ni, root, B<uint32_t>(s.mperm), ext_m);
         uint32_t* P = B<uint32_t>(s.tmp32);
         iota_kernel<<<g, kThreads, 0, st>>>(P, ni);
         for (int k = kExtWords - 1; k >= 0; --k) {
@@ -873,33 +1032,36 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error)
                                  B<uint32_t>(s.sv1), n, 63, s.hist, error));
         NBX_HIP(error, hipMemcpyAsync(P, s.sv0.p, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
         for (int k = 0; k < kExtWords; ++k) {
-            // ext_a[p*6+k] = ext_m[P[p]*6+k], via sk0 as staging
             gather_u64_kernel<<<g, kThreads, 0, st>>>(ext_m, P, B<uint64_t>(s.sk0), nullptr, ni,
                                                       kExtWords, k);
             NBX_HIP(error, hipMemcpy2DAsync(ext_a + k, kExtWords * sizeof(uint64_t), s.sk0.p,
                                             sizeof(uint64_t), sizeof(uint64_t), n,
                                             hipMemcpyDeviceToDevice, st));
         }
-        NBX_HIP(error, hipMemsetAsync(flags, 0, FLAG_COUNT * sizeof(unsigned), st));
+        NBX_HIP(error, hipMemsetAsync(flags, 0, kFlagWords * sizeof(unsigned), st));
         tree_order_kernel<<<g, kThreads, 0, st>>>(
-            P, B<uint32_t>(s.mperm), B<uint64_t>(s.tkey), ext_a, bx, by, bz, bm, ni,
+            P, B<uint32_t>(s.mperm), B<uint64_t>(s.tkey), ext_a, b4, ni,
             B<uint32_t>(s.tperm), B<uint32_t>(s.trank), B<uint64_t>(s.tword), ext_t,
             B<double>(s.sx), B<double>(s.sy), B<double>(s.sz), B<double>(s.sm), B<float4>(s.s4));
         delta_kernel<<<g, kThreads, 0, st>>>(B<uint64_t>(s.tword), ext_t, ni, B<int32_t>(s.delta),
-                                             B<uint32_t>(s.cnt), flags);
+                                             B<uint32_t>(s.cnt), flags, flags + kDCount);
         NBX_HIP(error, hipGetLastError());
+        NBX_HIP(error, hipMemcpyAsync(hflags, flags, kDOff * sizeof(unsigned),
+                                      hipMemcpyDeviceToHost, st));
+        NBX_HIP(error, hipStreamSynchronize(st));
         ext_tree = ext_t;
     }
-    NBX_TRY(exclusive_scan_u32(st, B<uint32_t>(s.cnt), B<uint32_t>(s.scan), n, s.scan_part, error));
-    uint32_t internal = 0;
-    NBX_HIP(error, hipMemcpyAsync(&internal, B<uint32_t>(s.scan) + n, sizeof(uint32_t),
-                                  hipMemcpyDeviceToHost, st));
-    NBX_HIP(error, hipMemcpyAsync(hflags, flags, sizeof(hflags), hipMemcpyDeviceToHost, st));
-    NBX_HIP(error, hipStreamSynchronize(st));
     if (hflags[FLAG_OVERFLOW] != 0) {
         NBX_FAIL(error, N_BODY_SIM_PRO_STATUS_OVERFLOW,
                  "Barnes-Hut tree exceeded maximum depth (duplicate positions?)");
     }
+    size_t internal = 0;
+    int deepest = -1;
+    for (int d = 0; d < kMaxLevels; ++d) {
+        internal += hflags[kDCount + d];
+        if (hflags[kDCount + d] != 0) deepest = d;
+    }
+    NBX_TRY(exclusive_scan_u32(st, B<uint32_t>(s.cnt), B<uint32_t>(s.scan), n, s.scan_part, error));
     NBX_HIP(error, hipEventRecord(e->ev[2], st));
 
     const size_t nodes = n + internal;
@@ -908,40 +1070,64 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error)
     NBX_TRY(s.ncz.need<double>(nodes, error));
     NBX_TRY(s.nm.need<double>(nodes, error));
     NBX_TRY(s.nskip.need<int32_t>(nodes, error));
-    NBX_TRY(s.nleaf.need<int32_t>(nodes, error));
-    NBX_TRY(s.nparent.need<int32_t>(nodes, error));
-    NBX_TRY(s.nchild.need<uint32_t>(nodes, error));
-    NBX_TRY(s.narrive.need<uint32_t>(nodes, error));
     NBX_TRY(s.ndepth.need<int32_t>(nodes, error));
     NBX_TRY(s.nstart.need<int32_t>(nodes, error));
     NBX_TRY(s.ncount.need<int32_t>(nodes, error));
     NBX_TRY(s.nminpos.need<int32_t>(nodes, error));
+    NBX_TRY(s.nlist.need<int>(internal + 1, error));
+    NBX_TRY(s.ninode.need<int>(internal + 1, error));
+    NBX_TRY(s.nkids.need<int>(8 * internal + 8, error));
+    NBX_HIP(error, hipMemsetAsync(s.nkids.p, 0xFF, (8 * internal + 8) * sizeof(int), st));
     NBX_TRY(s.n64.need<NodeF64>(nodes, error));
     NBX_TRY(s.n32.need<NodeF32>(nodes, error));
-    NBX_HIP(error, hipMemsetAsync(s.nchild.p, 0, nodes * sizeof(uint32_t), st));
-    NBX_HIP(error, hipMemsetAsync(s.narrive.p, 0, nodes * sizeof(uint32_t), st));
     NodeSoA nd = node_soa(s);
+    depth_offsets_kernel<<<1, 1, 0, st>>>(flags + kDCount, flags + kDOff);
     fill_kernel<<<g, kThreads, 0, st>>>(B<uint64_t>(s.tword), ext_tree, B<int32_t>(s.delta),
                                         B<uint32_t>(s.cnt), B<uint32_t>(s.scan), ni,
                                         B<double>(s.sx), B<double>(s.sy), B<double>(s.sz),
-                                        B<double>(s.sm), nd, flags);
+                                        B<double>(s.sm), s.root[3], nd, B<NodeF64>(s.n64),
+                                        B<NodeF32>(s.n32), flags + kDOff, flags + kDCursor,
+                                        B<int>(s.nlist), B<int>(s.ninode), B<int>(s.nkids));
     NBX_HIP(error, hipGetLastError());
     NBX_HIP(error, hipEventRecord(e->ev[3], st));
-    com_kernel<<<g, kThreads, 0, st>>>(B<uint32_t>(s.scan), B<uint32_t>(s.cnt),
-                                       B<uint32_t>(s.trank), ni, nd, B<double>(s.sx),
-                                       B<double>(s.sy), B<double>(s.sz), bx, by, bz);
-    NBX_HIP(error, hipGetLastError());
-    pack_kernel<<<grid_for(nodes, kThreads), kThreads, 0, st>>>(nd, static_cast<int>(nodes),
-                                                                 s.root[3], B<NodeF64>(s.n64),
-                                                                 B<NodeF32>(s.n32));
+    std::vector<unsigned> offs(kMaxLevels, 0);
+    for (int d = 0, run = 0; d < kMaxLevels; ++d) {
+        offs[d] = static_cast<unsigned>(run);
+        run += static_cast<int>(hflags[kDCount + d]);
+    }
+    // Deepest level first. Runs of small levels share one single-workgroup
+    // launch; a large level gets a full-grid launch of its own.
+    constexpr unsigned kSmallLevel = 512;
+    for (int d = deepest; d >= 0;) {
+        const unsigned count = hflags[kDCount + d];
+        if (count > kSmallLevel) {
+            com_level_kernel<<<grid_for(count, kThreads), kThreads, 0, st>>>(
+                B<int>(s.nlist) + offs[d], static_cast<int>(count), B<int>(s.ninode),
+                B<int>(s.nkids), nd, B<uint32_t>(s.trank),
+                B<double>(s.sx), B<double>(s.sy), B<double>(s.sz), b4, s.root[3],
+                B<NodeF64>(s.n64), B<NodeF32>(s.n32));
+            --d;
+            continue;
+        }
+        int lo = d;
+        while (lo - 1 >= 0 && hflags[kDCount + lo - 1] <= kSmallLevel) {
+            --lo;
+        }
+        com_small_levels_kernel<<<1, 1024, 0, st>>>(
+            B<int>(s.nlist), flags + kDOff, flags + kDCount, d, lo, B<int>(s.ninode),
+            B<int>(s.nkids), nd, B<uint32_t>(s.trank),
+            B<double>(s.sx), B<double>(s.sy), B<double>(s.sz), b4, s.root[3], B<NodeF64>(s.n64),
+            B<NodeF32>(s.n32));
+        d = lo - 1;
+    }
     NBX_HIP(error, hipGetLastError());
     NBX_HIP(error, hipEventRecord(e->ev[4], st));
-    NBX_HIP(error, hipMemcpyAsync(hflags, flags, sizeof(hflags), hipMemcpyDeviceToHost, st));
-    NBX_HIP(error, hipStreamSynchronize(st));
     s.n = n;
     s.internal = internal;
     s.nodes = nodes;
-    s.max_depth = static_cast<int>(hflags[FLAG_MAXDEPTH]);
+    // levels of the reference's compute_maximum_depth (barnes_hut.c:269-283):
+    // deepest leaf depth + 1, where the deepest leaf sits one below the deepest cell
+    s.max_depth = n == 1 ? 0 : deepest + 1;
     e->stats.node_count = nodes;
     e->stats.leaf_count = n;
     e->stats.internal_node_count = internal;
@@ -959,19 +1145,25 @@ NBodySimProStatus bh_walk(Engine* e, const NBodySimProGravity* g, bool fp64_exac
     NBX_HIP(error, hipMemsetAsync(s.counters.p, 0, 2 * sizeof(unsigned long long), st));
     NBX_HIP(error, hipEventRecord(e->ev[5], st));
     const double theta2 = e->theta * e->theta;
-    const unsigned grid = grid_for(s.n, kThreads);
+    const unsigned blocks = grid_for(s.n, kThreads);
+    const unsigned grid = (blocks + kXcds - 1) / kXcds * kXcds;
     if (fp64_exact) {
-        walk_kernel<true><<<grid, kThreads, 0, st>>>(
-            B<NodeF64>(s.n64), B<NodeF32>(s.n32), static_cast<int>(s.nodes), B<double>(s.sx),
-            B<double>(s.sy), B<double>(s.sz), B<float4>(s.s4), B<uint32_t>(s.tperm), ni,
-            g->gravitational_constant, g->softening_squared, theta2, e->ax, e->ay, e->az,
-            B<unsigned long long>(s.counters), nullptr);
+        walk_f64_kernel<<<blocks, kThreads, 0, st>>>(
+            B<NodeF64>(s.n64), static_cast<int>(s.nodes), B<double>(s.sx), B<double>(s.sy),
+            B<double>(s.sz), B<uint32_t>(s.tperm), ni, g->gravitational_constant,
+            g->softening_squared, theta2, e->ax, e->ay, e->az, B<unsigned long long>(s.counters));
+    } else if (features != nullptr) {
+        walk_f32_kernel<true><<<grid, kThreads, 0, st>>>(
+            B<NodeF32>(s.n32), static_cast<int>(s.nodes), B<float4>(s.s4), B<uint32_t>(s.tperm),
+            ni, static_cast<float>(g->gravitational_constant),
+            static_cast<float>(g->softening_squared), static_cast<float>(theta2), e->ax, e->ay,
+            e->az, B<unsigned long long>(s.counters), features);
     } else {
-        walk_kernel<false><<<grid, kThreads, 0, st>>>(
-            B<NodeF64>(s.n64), B<NodeF32>(s.n32), static_cast<int>(s.nodes), B<double>(s.sx),
-            B<double>(s.sy), B<double>(s.sz), B<float4>(s.s4), B<uint32_t>(s.tperm), ni,
-            g->gravitational_constant, g->softening_squared, theta2, e->ax, e->ay, e->az,
-            B<unsigned long long>(s.counters), features);
+        walk_f32_kernel<false><<<grid, kThreads, 0, st>>>(
+            B<NodeF32>(s.n32), static_cast<int>(s.nodes), B<float4>(s.s4), B<uint32_t>(s.tperm),
+            ni, static_cast<float>(g->gravitational_constant),
+            static_cast<float>(g->softening_squared), static_cast<float>(theta2), e->ax, e->ay,
+            e->az, B<unsigned long long>(s.counters), nullptr);
     }
     NBX_HIP(error, hipGetLastError());
     NBX_HIP(error, hipEventRecord(e->ev[6], st));

@@ -361,12 +361,12 @@ void nbx_engine_destroy(NBXEngine* engine) {
     }
     if (e->pos4 != nullptr) (void)hipFree(e->pos4);
     nbx::BHState& s = e->bh;
-    nbx::Buf* bufs[] = {&s.bx, &s.by, &s.bz, &s.bm, &s.partial, &s.counter, &s.flags, &s.mkey,
+    nbx::Buf* bufs[] = {&s.b4, &s.partial, &s.flags, &s.mkey,
                         &s.tkey, &s.sk0, &s.sk1, &s.sv0, &s.sv1, &s.hist, &s.mperm,
                         &s.mkey_sorted, &s.tperm, &s.trank, &s.tword, &s.ext, &s.delta, &s.cnt,
                         &s.scan, &s.scan_part, &s.sx, &s.sy, &s.sz, &s.sm, &s.s4, &s.n64, &s.n32,
-                        &s.ncx, &s.ncy, &s.ncz, &s.nm, &s.nskip, &s.nleaf, &s.nparent,
-                        &s.nchild, &s.narrive, &s.nstart, &s.ncount, &s.ndepth, &s.nminpos,
+                        &s.ncx, &s.ncy, &s.ncz, &s.nm, &s.nskip, &s.nlist, &s.nstart, &s.ncount,
+                        &s.ndepth, &s.nminpos, &s.ninode, &s.nkids,
                         &s.tmp32, &s.counters, &e->scratch, &e->scratch2};
     for (nbx::Buf* b : bufs) {
         nbx::free_buf(*b);
@@ -642,7 +642,12 @@ NBodySimProStatus nbx_engine_bh_export_nodes(NBXEngine* engine, double* rows,
     NBX_HIP(error, hipMemcpy(depth.data(), s.ndepth.p, nodes * 4, hipMemcpyDeviceToHost));
     NBX_HIP(error, hipMemcpy(start.data(), s.nstart.p, nodes * 4, hipMemcpyDeviceToHost));
     NBX_HIP(error, hipMemcpy(count.data(), s.ncount.p, nodes * 4, hipMemcpyDeviceToHost));
-    NBX_HIP(error, hipMemcpy(leaf.data(), s.nleaf.p, nodes * 4, hipMemcpyDeviceToHost));
+    {
+        std::vector<nbx::NodeF64> recs(nodes);
+        NBX_HIP(error, hipMemcpy(recs.data(), s.n64.p, nodes * sizeof(nbx::NodeF64),
+                                 hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < nodes; ++k) leaf[k] = recs[k].leaf;
+    }
     NBX_HIP(error, hipMemcpy(word.data(), s.tword.p, n * 8, hipMemcpyDeviceToHost));
     NBX_HIP(error, hipMemcpy(trank.data(), s.trank.p, n * 4, hipMemcpyDeviceToHost));
     if (s.tie_path) {

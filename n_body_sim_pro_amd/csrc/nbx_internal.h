@@ -90,10 +90,9 @@ struct BHState {
     bool mismatch_path = false;
     double root[4] = {0, 0, 0, 1};
     // build inputs (FP64 x, y, z, m): either the master arrays or FP32-rounded copies
-    Buf bx, by, bz, bm;
+    Buf b4;                   // (x, y, z, m) per particle in original order
     Buf partial;              // bbox partials + root
-    Buf counter;              // block-done counters
-    Buf flags;                // [0] mismatch, [1] tie, [2] overflow, [3] max depth
+    Buf flags;                // flags, per-depth counts, offsets and cursors
     Buf mkey, tkey;           // Morton / true-path keys in original order
     Buf sk0, sk1, sv0, sv1;   // sort ping-pong
     Buf hist;                 // sort histograms
@@ -112,7 +111,7 @@ struct BHState {
     Buf n64;                  // NodeF64
     Buf n32;                  // NodeF32
     Buf ncx, ncy, ncz, nm;    // FP64 centre of mass and mass per node
-    Buf nskip, nleaf, nparent, nchild, narrive, nstart, ncount, ndepth, nminpos;
+    Buf nskip, nstart, ncount, ndepth, nminpos, nlist, ninode, nkids;
     Buf tmp32;                // scratch permutation
     Buf counters;             // unsigned long long [approx, exact]
 };
