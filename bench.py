@@ -50,6 +50,9 @@ def parse():
     ap.add_argument("--ap-evals", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-all-pairs", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--ens-clusters", type=int, default=4096)
+    ap.add_argument("--ens-steps", type=int, default=50)
     return ap.parse_args()
 
 
@@ -145,6 +148,33 @@ def all_pairs(nbx, n, evals, eps, seed):
             "dtype": "f32 pairs, compensated f32 sums"}
 
 
+def ensemble(nbx, clusters, steps, eps, dt, seed):
+    """4096 Plummer clusters, sizes log-uniform in [64, 4096]: autotuned tile
+    shapes, one persistent launch per force evaluation, a hipGraph of `steps`
+    leapfrog steps; per-cluster FP64 energy drift over those steps."""
+    sizes = nbx.ensemble_sizes(clusters, lo=64, hi=4096, seed=seed)
+    d = nbx.ensemble_clusters(sizes, seed=seed)
+    with nbx.Ensemble(sizes) as ens:
+        ens.upload(d["pos"], d["vel"], d["m"])
+        ens.force(eps=eps)
+        tune = ens.autotune(eps=eps)
+        ens.upload(d["pos"], d["vel"], d["m"])
+        ens.force(eps=eps)
+        e0 = ens.energies(eps=eps).sum(1)
+        ms = ens.advance(dt, steps, graph_steps=steps, eps=eps)     # builds + replays the graph
+        ms2 = ens.advance(dt, steps, graph_steps=steps, eps=eps)    # replay only
+        e1 = ens.energies(eps=eps).sum(1)
+    n = sizes.astype(np.float64)
+    pairs = float(np.sum(n * (n - 1)))
+    drift = np.abs((e1 - e0) / e0)
+    return {"clusters": int(clusters), "particles": int(n.sum()), "steps": steps,
+            "ms_per_step": ms2 / steps, "first_replay_ms_per_step": ms / steps,
+            "pairs_per_step": pairs, "gint_s": pairs / (ms2 / steps * 1e-3) / 1e9,
+            "frac_fp32_peak_20flop": 20.0 * pairs / (ms2 / steps * 1e-3) / 1e12 / FP32_PEAK_TFLOPS,
+            "energy_drift_median": float(np.median(drift)), "energy_drift_max": float(drift.max()),
+            "drift_interval_steps": 2 * steps, "autotune": tune}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -215,6 +245,9 @@ def main():
     }
     if not args.no_all_pairs:
         out["all_pairs"] = all_pairs(nbx, args.ap_n, args.ap_evals, args.eps, args.seed)
+    if not args.no_ensemble:
+        out["ensemble"] = ensemble(nbx, args.ens_clusters, args.ens_steps, args.eps, args.dt,
+                                   args.seed)
     if not args.no_cpu_baseline:
         pos32 = d["pos"].astype(np.float32).astype(np.float64)
         m32 = d["m"].astype(np.float32).astype(np.float64)

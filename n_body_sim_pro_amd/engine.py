@@ -309,3 +309,137 @@ def ensemble_sizes(clusters, lo=64, hi=4096, seed=42):
     if st != 0:
         raise NBXError(st, "nbx_ensemble_sizes failed")
     return out
+
+
+class TileConfig(C.Structure):
+    """NBXTileConfig in include/nbx_ensemble.h."""
+    _fields_ = [("targets_per_lane", C.c_int * 8), ("blocks_per_cu", C.c_int),
+                ("source_tile", C.c_int)]
+
+
+def _ensemble_lib():
+    lib = load()
+    if getattr(lib, "_nbx_ensemble_bound", False):
+        return lib
+    vp, gp, ep = C.POINTER(View), C.POINTER(Gravity), C.POINTER(Error)
+    lib.nbx_ensemble_create.argtypes = [C.c_int, C.POINTER(C.c_uint32), C.c_size_t, ep]
+    lib.nbx_ensemble_create.restype = C.c_void_p
+    lib.nbx_ensemble_destroy.argtypes = [C.c_void_p]
+    lib.nbx_ensemble_particle_count.argtypes = [C.c_void_p]
+    lib.nbx_ensemble_particle_count.restype = C.c_size_t
+    lib.nbx_ensemble_upload.argtypes = [C.c_void_p, vp, ep]
+    lib.nbx_ensemble_download.argtypes = [C.c_void_p, vp, ep]
+    lib.nbx_ensemble_force.argtypes = [C.c_void_p, gp, ep]
+    lib.nbx_ensemble_advance.argtypes = [C.c_void_p, gp, C.c_double, C.c_int, C.c_int,
+                                         C.POINTER(C.c_double), ep]
+    lib.nbx_ensemble_energies.argtypes = [C.c_void_p, gp, C.POINTER(C.c_double), ep]
+    lib.nbx_ensemble_get_tile.argtypes = [C.c_void_p, C.POINTER(TileConfig)]
+    lib.nbx_ensemble_set_tile.argtypes = [C.c_void_p, C.POINTER(TileConfig), ep]
+    lib.nbx_ensemble_autotune.argtypes = [C.c_void_p, gp, C.c_char_p, C.c_char_p, C.c_size_t, ep]
+    lib._nbx_ensemble_bound = True
+    return lib
+
+
+class Ensemble:
+    """Batched ensemble of independent clusters (include/nbx_ensemble.h)."""
+
+    def __init__(self, sizes, device=0):
+        self.lib = _ensemble_lib()
+        self.sizes = np.ascontiguousarray(sizes, dtype=np.uint32)
+        err = Error()
+        self.h = self.lib.nbx_ensemble_create(
+            int(device), self.sizes.ctypes.data_as(C.POINTER(C.c_uint32)), self.sizes.size,
+            C.byref(err))
+        if not self.h:
+            raise NBXError(err.status, err.message.decode(errors="replace"))
+        self.offsets = np.concatenate([[0], np.cumsum(self.sizes.astype(np.int64))])
+        self.n = int(self.offsets[-1])
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.nbx_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def upload(self, pos, vel, mass):
+        p = np.ascontiguousarray(np.asarray(pos, dtype=np.float64).T).copy()
+        v = np.ascontiguousarray(np.asarray(vel, dtype=np.float64).T).copy()
+        m = np.ascontiguousarray(mass, dtype=np.float64).copy()
+        zeros = np.zeros((3, self.n))
+        view = make_view(self.n, px=p[0], py=p[1], pz=p[2], vx=v[0], vy=v[1], vz=v[2],
+                         ax=zeros[0], ay=zeros[1], az=zeros[2], m=m)
+        err = Error()
+        _check(self.lib.nbx_ensemble_upload(self.h, C.byref(view), C.byref(err)), err)
+
+    def download(self):
+        cols = {k: np.zeros(self.n) for k in ("px", "py", "pz", "vx", "vy", "vz", "ax", "ay",
+                                              "az", "m")}
+        err = Error()
+        _check(self.lib.nbx_ensemble_download(self.h, C.byref(make_view(self.n, **cols)),
+                                              C.byref(err)), err)
+        return {"pos": np.stack([cols["px"], cols["py"], cols["pz"]], 1),
+                "vel": np.stack([cols["vx"], cols["vy"], cols["vz"]], 1),
+                "acc": np.stack([cols["ax"], cols["ay"], cols["az"]], 1), "m": cols["m"]}
+
+    def force(self, G=1.0, eps=0.01):
+        err = Error()
+        g = gravity(G, eps)
+        _check(self.lib.nbx_ensemble_force(self.h, C.byref(g), C.byref(err)), err)
+
+    def advance(self, dt, steps, graph_steps=50, G=1.0, eps=0.01):
+        err = Error()
+        g = gravity(G, eps)
+        ms = C.c_double(0.0)
+        _check(self.lib.nbx_ensemble_advance(self.h, C.byref(g), float(dt), int(steps),
+                                             int(graph_steps), C.byref(ms), C.byref(err)), err)
+        return ms.value
+
+    def energies(self, G=1.0, eps=0.01):
+        out = np.zeros(2 * self.sizes.size)
+        err = Error()
+        g = gravity(G, eps)
+        _check(self.lib.nbx_ensemble_energies(self.h, C.byref(g),
+                                              out.ctypes.data_as(C.POINTER(C.c_double)),
+                                              C.byref(err)), err)
+        return out.reshape(-1, 2)
+
+    def tile(self):
+        t = TileConfig()
+        self.lib.nbx_ensemble_get_tile(self.h, C.byref(t))
+        return {"targets_per_lane": list(t.targets_per_lane), "blocks_per_cu": t.blocks_per_cu,
+                "source_tile": t.source_tile}
+
+    def set_tile(self, targets_per_lane, blocks_per_cu, source_tile):
+        t = TileConfig((C.c_int * 8)(*targets_per_lane), blocks_per_cu, source_tile)
+        err = Error()
+        _check(self.lib.nbx_ensemble_set_tile(self.h, C.byref(t), C.byref(err)), err)
+
+    def autotune(self, G=1.0, eps=0.01, cache_path=None):
+        err = Error()
+        g = gravity(G, eps)
+        buf = C.create_string_buffer(512)
+        _check(self.lib.nbx_ensemble_autotune(
+            self.h, C.byref(g), cache_path.encode() if cache_path else None, buf, 512,
+            C.byref(err)), err)
+        import json
+        return json.loads(buf.value.decode())
+
+
+def ensemble_clusters(sizes, seed=42):
+    """Concatenated Plummer-DF clusters (a=1, M=1 each), one seed per cluster."""
+    sizes = np.asarray(sizes, dtype=np.int64)
+    pos, vel, m = [], [], []
+    for k, n in enumerate(sizes):
+        d = plummer(int(n), seed=seed * 1_000_003 + k)
+        pos.append(d["pos"])
+        vel.append(d["vel"])
+        m.append(d["m"])
+    return {"pos": np.concatenate(pos), "vel": np.concatenate(vel), "m": np.concatenate(m)}
