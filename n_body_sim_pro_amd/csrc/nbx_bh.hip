@@ -796,6 +796,8 @@ __global__ __launch_bounds__(kThreads) void walk_f32_kernel(
     unsigned approx = 0, exact = 0;
     float ratio_sum = 0.f;
     float acx = 0.f, acy = 0.f, acz = 0.f;
+    float ccx = 0.f, ccy = 0.f, ccz = 0.f;   // accepted-cell part (FEATURES only)
+    float ratio_min = 0.f, quad = 0.f;       // FEATURES only
     int i = 0;
     while (i < nodes) {
         const NodeF32 cur = n32[__builtin_amdgcn_readfirstlane(i)];
@@ -830,7 +832,14 @@ __global__ __launch_bounds__(kThreads) void walk_f32_kernel(
                     acz = fmaf(s, dz, acz);
                     ++approx;
                     if (FEATURES) {
-                        ratio_sum += __builtin_amdgcn_sqrtf(d2) * __builtin_amdgcn_rsqf(cur.size2);
+                        ccx = fmaf(s, dx, ccx);
+                        ccy = fmaf(s, dy, ccy);
+                        ccz = fmaf(s, dz, ccz);
+                        const float ratio = __builtin_amdgcn_sqrtf(d2) * __builtin_amdgcn_rsqf(cur.size2);
+                        ratio_sum += ratio;
+                        ratio_min = approx == 1 ? ratio : fminf(ratio_min, ratio);
+                        // monopole truncation scale of this cell: (m / d^2) (size^2 / d^2)
+                        quad = fmaf(cur.m * cur.size2, __builtin_amdgcn_rcpf(d2 * d2), quad);
                     }
                     skip_until = cur.skip;
                 } else {
@@ -846,12 +855,14 @@ __global__ __launch_bounds__(kThreads) void walk_f32_kernel(
         ay[o] = static_cast<double>(G * acy);
         az[o] = static_cast<double>(G * acz);
         if (FEATURES) {
-            // per tree position: accepted cells, exact leaves, mean accepted distance/size
-            features[static_cast<size_t>(t) * 4 + 0] = static_cast<float>(approx);
-            features[static_cast<size_t>(t) * 4 + 1] = static_cast<float>(exact);
-            features[static_cast<size_t>(t) * 4 + 2] =
-                approx > 0 ? ratio_sum / static_cast<float>(approx) : 0.f;
-            features[static_cast<size_t>(t) * 4 + 3] = 0.f;
+            // 12 floats per tree position (G not applied): total sum xyz,
+            // accepted-cell sum xyz, accepted cells, exact leaf terms, mean and
+            // minimum accepted distance / cell size, sum of m size^2 / d^4, pad
+            float4* f = reinterpret_cast<float4*>(features + static_cast<size_t>(t) * 12);
+            f[0] = make_float4(acx, acy, acz, ccx);
+            f[1] = make_float4(ccy, ccz, static_cast<float>(approx), static_cast<float>(exact));
+            f[2] = make_float4(approx > 0 ? ratio_sum / static_cast<float>(approx) : 0.f,
+                               ratio_min, quad, 0.f);
         }
     }
     // wave totals: one atomic per wave

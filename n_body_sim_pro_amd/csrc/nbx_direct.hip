@@ -16,6 +16,7 @@
 //   lane owns R targets. Partial sums of T sources are folded into a
 //   per-target compensated (Kahan-Babuska) accumulator once per tile.
 #include "nbx_internal.h"
+#include "nbx_learned.h"
 
 namespace nbx {
 namespace {
@@ -24,15 +25,21 @@ constexpr int kExactTargets = 64;
 constexpr int kExactSources = 32;
 constexpr int kExactThreads = 256;
 
+// SUBSET: the block's targets are targets[k] for k = blockIdx.x * 64 + lane < count,
+// and the accelerations go to out[3k .. 3k+2].
+template <bool SUBSET>
 __global__ __launch_bounds__(kExactThreads) void direct_fp64_kernel(
     const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
     const double* __restrict__ m, double* __restrict__ ax, double* __restrict__ ay,
-    double* __restrict__ az, int n, double G, double eps2) {
+    double* __restrict__ az, int n, double G, double eps2, const int64_t* __restrict__ targets,
+    int count, double* __restrict__ out) {
     __shared__ double terms[3][kExactSources][kExactTargets];
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
-    const int i = blockIdx.x * kExactTargets + lane;
-    const int ic = i < n ? i : n - 1;
+    const int k = blockIdx.x * kExactTargets + lane;
+    const bool valid = SUBSET ? k < count : k < n;
+    const int i = !valid ? -1 : (SUBSET ? static_cast<int>(targets[k]) : k);
+    const int ic = valid ? i : n - 1;
     const double xi = px[ic], yi = py[ic], zi = pz[ic];
     double acc = 0.0;
     for (int j0 = 0; j0 < n; j0 += kExactSources) {
@@ -65,9 +72,13 @@ __global__ __launch_bounds__(kExactThreads) void direct_fp64_kernel(
         }
         __syncthreads();
     }
-    if (i < n && wave < 3) {
-        double* out = wave == 0 ? ax : (wave == 1 ? ay : az);
-        out[i] = acc;
+    if (valid && wave < 3) {
+        if (SUBSET) {
+            out[3 * static_cast<size_t>(k) + wave] = acc;
+        } else {
+            double* o = wave == 0 ? ax : (wave == 1 ? ay : az);
+            o[i] = acc;
+        }
     }
 }
 
@@ -174,9 +185,9 @@ NBodySimProStatus direct_fp64(Engine* e, const NBodySimProGravity* g, NBodySimPr
         return N_BODY_SIM_PRO_STATUS_OK;
     }
     NBX_HIP(error, hipEventRecord(e->ev[0], e->stream));
-    direct_fp64_kernel<<<grid_for(n, kExactTargets), kExactThreads, 0, e->stream>>>(
+    direct_fp64_kernel<false><<<grid_for(n, kExactTargets), kExactThreads, 0, e->stream>>>(
         e->px, e->py, e->pz, e->m, e->ax, e->ay, e->az, static_cast<int>(n),
-        g->gravitational_constant, g->softening_squared);
+        g->gravitational_constant, g->softening_squared, nullptr, 0, nullptr);
     NBX_HIP(error, hipGetLastError());
     NBX_HIP(error, hipEventRecord(e->ev[1], e->stream));
     NBX_HIP(error, hipEventSynchronize(e->ev[1]));
@@ -226,3 +237,42 @@ NBodySimProStatus direct_fp32(Engine* e, const NBodySimProGravity* g, NBodySimPr
 }
 
 }  // namespace nbx
+
+extern "C" NBodySimProStatus nbx_engine_direct_fp64_targets(NBXEngine* engine,
+                                                            const NBodySimProGravity* gravity,
+                                                            const int64_t* targets, size_t count,
+                                                            double* out, NBodySimProError* error) {
+    if (engine == nullptr || gravity == nullptr || (count > 0 && (targets == nullptr || out == nullptr))) {
+        NBX_FAIL(error, N_BODY_SIM_PRO_STATUS_INVALID_ARGUMENT,
+                 "engine, gravity, targets and out are required");
+    }
+    nbx::Engine* e = &engine->e;
+    if (count == 0) {
+        return N_BODY_SIM_PRO_STATUS_OK;
+    }
+    if (count > 0x7FFFFFFFull) {
+        NBX_FAIL(error, N_BODY_SIM_PRO_STATUS_INVALID_ARGUMENT, "too many targets");
+    }
+    for (size_t k = 0; k < count; ++k) {
+        if (targets[k] < 0 || static_cast<size_t>(targets[k]) >= e->n) {
+            NBX_FAIL(error, N_BODY_SIM_PRO_STATUS_INVALID_ARGUMENT, "target index out of range");
+        }
+    }
+    NBX_HIP(error, hipSetDevice(e->device));
+    const size_t tbytes = (count * sizeof(int64_t) + 255) / 256 * 256;
+    NBX_TRY(e->scratch.need<unsigned char>(tbytes + 3 * count * sizeof(double), error));
+    int64_t* dt = e->scratch.as<int64_t>();
+    double* dout = reinterpret_cast<double*>(e->scratch.as<unsigned char>() + tbytes);
+    NBX_HIP(error, hipMemcpyAsync(dt, targets, count * sizeof(int64_t), hipMemcpyHostToDevice,
+                                  e->stream));
+    nbx::direct_fp64_kernel<true>
+        <<<nbx::grid_for(count, nbx::kExactTargets), nbx::kExactThreads, 0, e->stream>>>(
+            e->px, e->py, e->pz, e->m, nullptr, nullptr, nullptr, static_cast<int>(e->n),
+            gravity->gravitational_constant, gravity->softening_squared, dt,
+            static_cast<int>(count), dout);
+    NBX_HIP(error, hipGetLastError());
+    NBX_HIP(error, hipMemcpyAsync(out, dout, 3 * count * sizeof(double), hipMemcpyDeviceToHost,
+                                  e->stream));
+    NBX_HIP(error, hipStreamSynchronize(e->stream));
+    return N_BODY_SIM_PRO_STATUS_OK;
+}

@@ -1,0 +1,294 @@
+"""Learned Barnes-Hut correction on the GPU (include/nbx_learned.h).
+
+- the v_mfma_f32_16x16x16_bf16 lane layout the kernels assume;
+- MLP inference and gradients against the CPU oracle (oracle/mlp_oracle.py);
+- training lowers the loss; the feature walk against a numpy restatement of
+  the walk over the exported tree; the correction kernel against the host
+  formula; without a model (or with a zero model) hip_bh_learned is plain BH;
+- the committed model lowers the theta = 1.0 error on held-out snapshots.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+sys.path.insert(0, str(ROOT / "oracle"))
+import mlp_oracle as mo  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def learned(nbx):
+    from n_body_sim_pro_amd import learned as L
+    return L
+
+
+def _random_model(seed):
+    rng = np.random.default_rng(seed)
+    p = mo.init_params(seed)
+    p[mo.OFF_W3:mo.OFF_W3 + 3 * mo.H] *= 10.0
+    p[mo.OFF_B1:mo.OFF_W2] = rng.normal(0, 0.1, mo.H)
+    p[mo.OFF_B2:mo.OFF_W3] = rng.normal(0, 0.1, mo.H)
+    p[mo.OFF_B3:mo.OFF_B3 + 3] = rng.normal(0, 0.1, 3)
+    mean = rng.normal(0, 0.2, mo.F).astype(np.float32)
+    std = rng.uniform(0.5, 2.0, mo.F).astype(np.float32)
+    return p, mean, std
+
+
+def _feats(n, seed):
+    rng = np.random.default_rng(seed)
+    f = rng.normal(0, 1.5, (n, mo.F)).astype(np.float32)
+    f[:, 8:] = 0.0
+    return f
+
+
+def test_mfma_bf16_lane_layout(nbx, learned):
+    rng = np.random.default_rng(0)
+    a = rng.normal(0, 1, (16, 16)).astype(np.float32)
+    b = rng.normal(0, 1, (16, 16)).astype(np.float32)
+    d = learned.probe_mfma(a, b)
+    ref = mo.bf16(a).astype(np.float64) @ mo.bf16(b).astype(np.float64)
+    assert np.allclose(d, ref, rtol=1e-6, atol=1e-5 * np.abs(ref).max())
+    # a transposed operand would not pass
+    assert not np.allclose(d, mo.bf16(a).astype(np.float64) @ mo.bf16(b).T.astype(np.float64),
+                           rtol=1e-3)
+
+
+@pytest.mark.parametrize("n", [1, 17, 1000, 4096])
+def test_forward_matches_oracle(learned, n):
+    p, mean, std = _random_model(1)
+    f = _feats(n, 2)
+    with learned.Mlp() as m:
+        m.set_params(p, mean, std, 1.0)
+        y = m.forward(f)
+    yo, _ = mo.forward(p, mean, std, f)
+    diff = np.abs(y - yo)
+    scale = np.abs(yo).max()
+    print(f"n={n} forward max |diff| {diff.max():.3e} median {np.median(diff):.3e} scale {scale:.3e}")
+    assert np.median(diff) <= 1e-5 * scale + 1e-7
+    assert diff.max() <= 2e-2 * scale
+
+
+@pytest.mark.parametrize("n", [1, 17, 1000, 5000])
+def test_gradients_match_oracle(learned, n):
+    p, mean, std = _random_model(3)
+    f = _feats(n, 4)
+    t = np.random.default_rng(5).normal(0, 1, (n, 3)).astype(np.float32)
+    with learned.Mlp() as m:
+        m.set_params(p, mean, std, 1.0)
+        g, L = m.gradients(f, t)
+    go, Lo = mo.gradients(p, mean, std, f, t)
+    rel = np.linalg.norm(g - go) / np.linalg.norm(go)
+    print(f"n={n} loss {L:.6f} oracle {Lo:.6f} grad rel norm diff {rel:.3e}")
+    assert L == pytest.approx(Lo, rel=1e-3)
+    assert rel < 1e-2
+    for lo, hi in ((mo.OFF_W1, mo.OFF_B1), (mo.OFF_W2, mo.OFF_B2), (mo.OFF_W3, mo.OFF_B3)):
+        blk = np.linalg.norm(g[lo:hi] - go[lo:hi]) / np.linalg.norm(go[lo:hi])
+        assert blk < 2e-2, (lo, blk)
+    assert not g[mo.OFF_W3 + 3 * mo.H:mo.OFF_B3].any() and not g[mo.OFF_B3 + 3:].any()
+
+
+def test_training_lowers_loss(learned):
+    n = 32768
+    f = _feats(n, 6)
+    t = np.stack([np.tanh(f[:, 0] + 0.5 * f[:, 1]), 0.3 * f[:, 2] * f[:, 3],
+                  0.2 * np.sin(f[:, 4])], 1).astype(np.float32)
+    with learned.Mlp() as m:
+        m.set_params(learned.init_params(0), np.zeros(16), np.ones(16), 1.0)
+        _, l0 = m.gradients(f, t)
+        m.load_dataset(f, t)
+        hist = m.train(15, 512, 3e-3, 1)
+        _, l1 = m.gradients(f, t)
+    print("loss history", np.round(hist, 4), "start", l0, "end", l1)
+    assert l1 < 0.15 * l0
+    assert hist[-1] < hist[0]
+
+
+# ---- feature walk: numpy restatement over the exported tree ----
+
+def _walk_features_numpy(rows, perm, pos32, m32, root_h, theta, eps2):
+    """Per-particle walk statistics over pre-order node rows, in FP64."""
+    nodes = len(rows)
+    depth = rows[:, 0].astype(int)
+    skip = np.empty(nodes, dtype=int)
+    stack = []
+    for k in range(nodes):
+        while stack and depth[stack[-1]] >= depth[k]:
+            skip[stack.pop()] = k
+        stack.append(k)
+    for k in stack:
+        skip[k] = nodes
+    n = len(pos32)
+    q = pos32.astype(np.float64)
+    leaf_particle = np.where(rows[:, 10] >= 0, perm[np.maximum(rows[:, 10], 0).astype(int)], -1)
+    skip_until = np.zeros(n, dtype=int)
+    tot = np.zeros((n, 3))
+    cells = np.zeros((n, 3))
+    n_cells = np.zeros(n)
+    n_leaves = np.zeros(n)
+    ratio_sum = np.zeros(n)
+    ratio_min = np.full(n, np.inf)
+    quad = np.zeros(n)
+    leaf_depth = np.zeros(n, dtype=int)
+    idx = np.arange(n)
+    theta2 = theta * theta
+    for k in range(nodes):
+        active = skip_until <= k
+        com = rows[k, 7:10]
+        d = com[None, :] - q
+        d2 = (d * d).sum(1)
+        if leaf_particle[k] >= 0:
+            leaf_depth[leaf_particle[k]] = depth[k]
+            use = active & (idx != leaf_particle[k])
+            r2 = d2 + eps2
+            s = rows[k, 6] / (r2 * np.sqrt(r2))
+            tot[use] += s[use, None] * d[use]
+            n_leaves[use] += 1
+        else:
+            size2 = 4.0 * rows[k, 4] ** 2
+            acc = active & (d2 > 0) & (size2 < theta2 * d2)
+            r2 = d2 + eps2
+            s = rows[k, 6] / (r2 * np.sqrt(r2))
+            tot[acc] += s[acc, None] * d[acc]
+            cells[acc] += s[acc, None] * d[acc]
+            n_cells[acc] += 1
+            ratio = np.sqrt(d2) / np.sqrt(size2)
+            ratio_sum[acc] += ratio[acc]
+            ratio_min[acc] = np.minimum(ratio_min[acc], ratio[acc])
+            quad[acc] += rows[k, 6] * size2 / (d2[acc] ** 2)
+            skip_until[acc] = skip[k]
+    a = np.linalg.norm(tot, axis=1)
+    e1 = tot / a[:, None]
+    c_par = (cells * e1).sum(1)
+    perp = cells - c_par[:, None] * e1
+    c_perp = np.linalg.norm(perp, axis=1)
+    f = np.zeros((n, 16))
+    f[:, 0] = c_par / a
+    f[:, 1] = c_perp / a
+    f[:, 2] = np.log2(1 + n_cells)
+    f[:, 3] = np.log2(1 + n_leaves)
+    f[:, 4] = np.where(n_cells > 0, ratio_sum / np.maximum(n_cells, 1), 0)
+    f[:, 5] = np.where(n_cells > 0, ratio_min, 0)
+    f[:, 6] = np.log2(quad / a + 2.0 ** -40)
+    f[:, 7] = np.log2(m32) - 3 * (np.log2(2 * root_h) - leaf_depth)
+    e2 = perp / np.where(c_perp > 0, c_perp, 1)[:, None]
+    return f, e1, e2, c_perp / a, n_cells, n_leaves
+
+
+def test_feature_walk_matches_numpy_restatement(nbx):
+    d = nbx.plummer(3000, seed=11)
+    theta, eps = 1.0, 0.01
+    with nbx.Engine(3000, theta=theta) as e:
+        e.upload(d["pos"], d["m"])
+        feats, frames = e.bh_features(1.0, eps, frames=True)
+        acc_feat = e.accelerations()
+        stats = e.stats()
+        rows = e.bh_export_nodes()
+        root, _, perm = e.bh_morton()
+        e.compute_force("hip_barnes_hut", 1.0, eps)
+        acc_plain = e.accelerations()
+    # the feature walk leaves exactly the plain walk's accelerations
+    assert np.array_equal(acc_feat, acc_plain)
+    # per-particle counts add up to the walk's own totals
+    cells = np.rint(np.exp2(feats[:, 2].astype(np.float64)) - 1)
+    leaves = np.rint(np.exp2(feats[:, 3].astype(np.float64)) - 1)
+    assert cells.sum() == stats["accepted_approximations"]
+    assert leaves.sum() == stats["exact_interactions"]
+    # frames are orthonormal and right-handed
+    g = np.einsum("nij,nkj->nik", frames.astype(np.float64), frames.astype(np.float64))
+    assert np.abs(g - np.eye(3)).max() < 1e-5
+    assert np.abs(np.linalg.det(frames.astype(np.float64)) - 1).max() < 1e-5
+    pos32 = d["pos"].astype(np.float32)
+    m32 = d["m"].astype(np.float32)
+    f, e1, e2, cperp, nc, nl = _walk_features_numpy(rows, perm, pos32, m32, root[3], theta,
+                                                    eps * eps)
+    same = (nc == cells) & (nl == leaves)
+    print(f"particles with identical interaction lists: {same.mean():.4f}")
+    assert same.mean() > 0.99
+    fs, fg = f[same], feats[same].astype(np.float64)
+    assert np.abs(fs[:, 0] - fg[:, 0]).max() < 1e-4
+    assert np.abs(fs[:, 1] - fg[:, 1]).max() < 1e-4
+    assert np.array_equal(np.rint(np.exp2(fs[:, 2:4]) - 1), np.rint(np.exp2(fg[:, 2:4]) - 1))
+    assert np.allclose(fs[:, 4:6], fg[:, 4:6], rtol=1e-5, atol=1e-6)
+    assert np.abs(fs[:, 6] - fg[:, 6]).max() < 1e-3
+    assert np.abs(fs[:, 7] - fg[:, 7]).max() < 1e-4
+    assert not fg[:, 8:].any()
+    assert np.abs(frames[same, 0].astype(np.float64) - e1[same]).max() < 1e-5
+    ok = same & (cperp > 1e-3)
+    assert np.abs(frames[ok, 1].astype(np.float64) - e2[ok]).max() < 1e-3
+
+
+def test_correction_kernel_matches_host_formula(nbx, learned):
+    d = nbx.plummer(20000, seed=12)
+    p, mean, std = _random_model(7)
+    out_scale = 50.0
+    with nbx.Engine(20000, theta=1.0) as e, learned.Mlp() as m:
+        m.set_params(p, mean, std, out_scale)
+        e.upload(d["pos"], d["m"])
+        feats, frames = e.bh_features(1.0, 0.01, frames=True)
+        a_bh = e.accelerations()
+        y = m.forward(feats)
+        e.set_mlp(m)
+        e.compute_force("hip_bh_learned", 1.0, 0.01)
+        a_l = e.accelerations()
+    a = np.linalg.norm(a_bh, axis=1)
+    want = a_bh + (a / out_scale)[:, None] * np.einsum("ni,nij->nj", y.astype(np.float64),
+                                                        frames.astype(np.float64))
+    corr = np.linalg.norm(want - a_bh, axis=1)
+    assert np.median(corr / a) > 1e-3                    # the correction is not trivial
+    err = np.linalg.norm(a_l - want, axis=1) / a
+    print(f"apply vs host formula: max {err.max():.3e}")
+    assert err.max() < 1e-5
+
+
+def test_without_model_hip_bh_learned_is_plain_bh(nbx, learned):
+    d = nbx.two_galaxy(30000, seed=3)
+    with nbx.Engine(30000, theta=1.0) as e:
+        e.upload(d["pos"], d["m"])
+        e.compute_force("hip_barnes_hut", 1.0, 0.01)
+        plain = e.accelerations()
+        e.compute_force("hip_bh_learned", 1.0, 0.01)
+        assert np.array_equal(e.accelerations(), plain)
+        with learned.Mlp() as m:   # a zero model changes nothing either
+            m.set_params(np.zeros(m.n_params), np.zeros(16), np.ones(16), 1.0)
+            e.set_mlp(m)
+            e.compute_force("hip_bh_learned", 1.0, 0.01)
+            assert np.array_equal(e.accelerations(), plain)
+            e.set_mlp(None)
+        e.compute_force("hip_bh_learned", 1.0, 0.01)
+        assert np.array_equal(e.accelerations(), plain)
+
+
+def _errors(nbx, learned, d, mlp, eps=0.01):
+    n = len(d["m"])
+    with nbx.Engine(n, theta=1.0) as e:
+        e.upload(d["pos"], d["m"])
+        e.compute_force("hip_direct_fp64", 1.0, eps)
+        ref = e.accelerations()
+        e.compute_force("hip_barnes_hut", 1.0, eps)
+        plain = e.accelerations()
+        e.set_mlp(mlp)
+        e.compute_force("hip_bh_learned", 1.0, eps)
+        corr = e.accelerations()
+        e.set_mlp(None)
+    r = np.linalg.norm(ref, axis=1)
+    return (np.linalg.norm(plain - ref, axis=1) / r, np.linalg.norm(corr - ref, axis=1) / r)
+
+
+@pytest.mark.parametrize("kind,seed", [("plummer", 101), ("disk", 102), ("two_galaxy", 103)])
+def test_committed_model_lowers_theta1_error(nbx, learned, kind, seed):
+    """Held-out snapshots (seeds outside the training set; the two-galaxy
+    system is not in it at all): median error of theta = 1.0 + model vs
+    plain theta = 1.0."""
+    assert learned.WEIGHTS.exists(), f"{learned.WEIGHTS} is missing"
+    gen = {"plummer": nbx.plummer, "disk": nbx.disk_galaxy, "two_galaxy": nbx.two_galaxy}[kind]
+    d = gen(65536, seed=seed)
+    mlp = learned.Mlp.load(learned.WEIGHTS)
+    plain, corr = _errors(nbx, learned, d, mlp)
+    mlp.close()
+    print(f"{kind}: median error plain {np.median(plain):.4e} learned {np.median(corr):.4e} "
+          f"p90 plain {np.quantile(plain, 0.9):.4e} learned {np.quantile(corr, 0.9):.4e}")
+    assert np.median(corr) < 0.9 * np.median(plain)
