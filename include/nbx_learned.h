@@ -11,15 +11,19 @@
  * component; it only counts accepted cells during its walk
  * (src/c/barnes_hut/barnes_hut.c:611), which is one of the features.
  *
- * Features (raw, before standardisation; all frame-free), per particle:
- *   0  accepted-cell force along e1 / |a_BH|
- *   1  |accepted-cell force perpendicular to e1| / |a_BH|
- *   2  log2(1 + accepted cells)          3  log2(1 + exact leaf terms)
- *   4  mean accepted distance / cell edge
- *   5  minimum accepted distance / cell edge (0 without accepted cells)
- *   6  log2(sum over accepted cells of m edge^2 / d^4 / |a_BH| + 2^-40)
- *   7  log2 local density: own mass / (leaf cell edge)^3
- *   8-15 zero
+ * Features (raw, before standardisation; all frame-free), per particle, with
+ * W = sum over accepted cells of m edge^2 / r^6 d d^T (d from the particle to
+ * the cell's centre of mass, r^2 = d.d + eps^2: the monopole truncation scale
+ * of each cell spread over its direction) and a = the walk's acceleration:
+ *   0  rms accepted distance / cell edge     1  minimum accepted distance / edge
+ *   2  log2(tr W / |a| + 2^-40)              3  log2 local density: own mass /
+ *                                               (leaf cell edge)^3
+ *   4  e1.W.e1 / tr W                        5  e2.W.e2 / tr W
+ *   6  smallest eigenvalue of W / tr W       7  largest eigenvalue of W / tr W
+ *   8  log2(1 + accepted cells)              9  log2(1 + exact leaf terms)
+ *   10 accepted-cell force along e1 / |a|   11  its perpendicular part / |a|
+ *   12-15 zero
+ * A feature whose std is +inf standardises to 0, i.e. the model ignores it.
  * Parameter layout (float, nbx_mlp_param_count() values):
  *   W1[64][16], b1[64], W2[64][64], b2[64], W3[16][64] (rows 3..15 unused,
  *   kept zero), b3[16] (entries 3..15 unused).

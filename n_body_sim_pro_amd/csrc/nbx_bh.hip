@@ -571,7 +571,7 @@ __global__ __launch_bounds__(kThreads) void fill_kernel(
     b.size2 = 0.f;
     b.skip = li + 1;
     b.leaf = t;
-    b.depth = depth;
+    b.inv_size2 = 0.f;
     n32[li] = b;
 }
 
@@ -654,7 +654,7 @@ __device__ inline void com_node(int iid, const int* __restrict__ inode,
     b.size2 = static_cast<float>(4.0 * h * h);
     b.skip = end;
     b.leaf = -1;
-    b.depth = depth;
+    b.inv_size2 = 1.f / b.size2;
     n32[p] = b;
 }
 
@@ -794,10 +794,12 @@ __global__ __launch_bounds__(kThreads) void walk_f32_kernel(
     const float4 q = s4[valid ? t : n - 1];
     int skip_until = valid ? 0 : 0x7FFFFFFF;
     unsigned approx = 0, exact = 0;
-    float ratio_sum = 0.f;
     float acx = 0.f, acy = 0.f, acz = 0.f;
-    float ccx = 0.f, ccy = 0.f, ccz = 0.f;   // accepted-cell part (FEATURES only)
-    float ratio_min = 0.f, quad = 0.f;       // FEATURES only
+    // FEATURES only: accepted-cell force, error-weighted direction tensor W,
+    // sum and minimum of (distance / cell edge)^2
+    float ccx = 0.f, ccy = 0.f, ccz = 0.f;
+    float wxx = 0.f, wyy = 0.f, wzz = 0.f, wxy = 0.f, wxz = 0.f, wyz = 0.f;
+    float q2_sum = 0.f, q2_min = 3.0e38f;
     int i = 0;
     while (i < nodes) {
         const NodeF32 cur = n32[__builtin_amdgcn_readfirstlane(i)];
@@ -835,11 +837,19 @@ __global__ __launch_bounds__(kThreads) void walk_f32_kernel(
                         ccx = fmaf(s, dx, ccx);
                         ccy = fmaf(s, dy, ccy);
                         ccz = fmaf(s, dz, ccz);
-                        const float ratio = __builtin_amdgcn_sqrtf(d2) * __builtin_amdgcn_rsqf(cur.size2);
-                        ratio_sum += ratio;
-                        ratio_min = approx == 1 ? ratio : fminf(ratio_min, ratio);
-                        // monopole truncation scale of this cell: (m / d^2) (size^2 / d^2)
-                        quad = fmaf(cur.m * cur.size2, __builtin_amdgcn_rcpf(d2 * d2), quad);
+                        // monopole truncation scale m size^2 / r^4 spread over
+                        // the direction d / r: w = m size^2 / r^6
+                        const float w = (s * cur.size2) * (inv * inv) * inv;
+                        const float wx = w * dx, wy = w * dy;
+                        wxx = fmaf(wx, dx, wxx);
+                        wyy = fmaf(wy, dy, wyy);
+                        wzz = fmaf(w * dz, dz, wzz);
+                        wxy = fmaf(wx, dy, wxy);
+                        wxz = fmaf(wx, dz, wxz);
+                        wyz = fmaf(wy, dz, wyz);
+                        const float q2 = d2 * cur.inv_size2;
+                        q2_sum += q2;
+                        q2_min = fminf(q2_min, q2);
                     }
                     skip_until = cur.skip;
                 } else {
@@ -850,19 +860,21 @@ __global__ __launch_bounds__(kThreads) void walk_f32_kernel(
         }
     }
     if (valid) {
-        const uint32_t o = tperm[t];
-        ax[o] = static_cast<double>(G * acx);
-        ay[o] = static_cast<double>(G * acy);
-        az[o] = static_cast<double>(G * acz);
-        if (FEATURES) {
-            // 12 floats per tree position (G not applied): total sum xyz,
-            // accepted-cell sum xyz, accepted cells, exact leaf terms, mean and
-            // minimum accepted distance / cell size, sum of m size^2 / d^4, pad
-            float4* f = reinterpret_cast<float4*>(features + static_cast<size_t>(t) * 12);
-            f[0] = make_float4(acx, acy, acz, ccx);
-            f[1] = make_float4(ccy, ccz, static_cast<float>(approx), static_cast<float>(exact));
-            f[2] = make_float4(approx > 0 ? ratio_sum / static_cast<float>(approx) : 0.f,
-                               ratio_min, quad, 0.f);
+        if (!FEATURES) {
+            // (with FEATURES the consumer of the record scatters G * sum itself)
+            const uint32_t o = tperm[t];
+            ax[o] = static_cast<double>(G * acx);
+            ay[o] = static_cast<double>(G * acy);
+            az[o] = static_cast<double>(G * acz);
+        } else {
+            // kWalkRecord floats per tree position (G not applied): total sum
+            // xyz, accepted cells; accepted-cell sum xyz, exact leaf terms;
+            // W xx yy zz xy; W xz yz, sum and minimum of (d / edge)^2
+            float4* f = reinterpret_cast<float4*>(features + static_cast<size_t>(t) * kWalkRecord);
+            f[0] = make_float4(acx, acy, acz, static_cast<float>(approx));
+            f[1] = make_float4(ccx, ccy, ccz, static_cast<float>(exact));
+            f[2] = make_float4(wxx, wyy, wzz, wxy);
+            f[3] = make_float4(wxz, wyz, q2_sum, approx > 0 ? q2_min : 0.f);
         }
     }
     // wave totals: one atomic per wave

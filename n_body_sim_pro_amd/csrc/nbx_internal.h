@@ -12,6 +12,8 @@
 namespace nbx {
 
 constexpr int kWave = 64;
+// Floats per query of the FP32 walk's statistics record (learned correction).
+constexpr int kWalkRecord = 16;
 
 // Fills the optional error record (reference contract: core/status.h:37-49).
 void set_error(NBodySimProError* error, NBodySimProStatus status, const char* file, int line,
@@ -63,13 +65,14 @@ struct Buf {
     }
 };
 
-// FP32 walk node: centre of mass, mass, opening size^2 (4h^2), pre-order skip, leaf.
+// FP32 walk node: centre of mass, mass, opening size^2 (4h^2), pre-order skip,
+// leaf, and 1 / size^2 (walk statistics of the learned correction; 0 for leaves).
 struct alignas(32) NodeF32 {
     float cx, cy, cz, m;
     float size2;
     int32_t skip;
     int32_t leaf;   // tree position of the leaf particle, -1 for an internal node
-    int32_t depth;
+    float inv_size2;
 };
 
 // FP64 walk node (exact reference arithmetic).

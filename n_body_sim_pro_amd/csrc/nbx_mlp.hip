@@ -91,6 +91,7 @@ __device__ inline bf16x4 frag_cols(const float* __restrict__ M, int ld, int r0, 
     return pack(M[k * ld + i], M[(k + 1) * ld + i], M[(k + 2) * ld + i], M[(k + 3) * ld + i]);
 }
 
+
 // Accumulator-layout bias: rows r0 + 4*(lane/16) + v.
 __device__ inline f32x4 bias4(const float* __restrict__ b, int r0, int lane) {
     const float* p = b + r0 + 4 * (lane >> 4);
@@ -102,6 +103,7 @@ struct Weights {
     bf16x4 w1[4];       // A of layer 1: W1 rows (hidden tile mt), cols = features
     bf16x4 w2[4][4];    // A of layer 2: W2[mt][ks]
     bf16x4 w3[4];       // A of layer 3: W3 (outputs) x hidden k-step
+    f32x4 b1[4], b2[4], b3;   // biases in accumulator layout
 };
 
 __device__ inline void load_weights(const float* __restrict__ P, int lane, Weights& w) {
@@ -113,7 +115,10 @@ __device__ inline void load_weights(const float* __restrict__ P, int lane, Weigh
             w.w2[mt][ks] = frag_rows(P + OFF_W2, kH, 16 * mt, 16 * ks, lane);
         }
         w.w3[mt] = frag_rows(P + OFF_W3, kH, 0, 16 * mt, lane);
+        w.b1[mt] = bias4(P + OFF_B1, 16 * mt, lane);
+        w.b2[mt] = bias4(P + OFF_B2, 16 * mt, lane);
     }
+    w.b3 = bias4(P + OFF_B3, 0, lane);
 }
 
 // Standardised BF16 input fragment: sample (lane % 16) of the group, features
@@ -137,17 +142,16 @@ struct Forward {
     f32x4 y;
 };
 
-__device__ inline void forward(const float* __restrict__ P, const Weights& w, bf16x4 xb, int lane,
-                               Forward& f) {
+__device__ inline void forward(const Weights& w, bf16x4 xb, Forward& f) {
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) {
-        f.z1[mt] = mfma(w.w1[mt], xb, bias4(P + OFF_B1, 16 * mt, lane));
+        f.z1[mt] = mfma(w.w1[mt], xb, w.b1[mt]);
         f.h1[mt] = pack(f.z1[mt][0] * sigmoidf_(f.z1[mt][0]), f.z1[mt][1] * sigmoidf_(f.z1[mt][1]),
                         f.z1[mt][2] * sigmoidf_(f.z1[mt][2]), f.z1[mt][3] * sigmoidf_(f.z1[mt][3]));
     }
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) {
-        f32x4 acc = bias4(P + OFF_B2, 16 * mt, lane);
+        f32x4 acc = w.b2[mt];
 #pragma unroll
         for (int ks = 0; ks < 4; ++ks) {
             acc = mfma(w.w2[mt][ks], f.h1[ks], acc);
@@ -156,7 +160,7 @@ __device__ inline void forward(const float* __restrict__ P, const Weights& w, bf
         f.h2[mt] = pack(acc[0] * sigmoidf_(acc[0]), acc[1] * sigmoidf_(acc[1]),
                         acc[2] * sigmoidf_(acc[2]), acc[3] * sigmoidf_(acc[3]));
     }
-    f32x4 y = bias4(P + OFF_B3, 0, lane);
+    f32x4 y = w.b3;
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks) {
         y = mfma(w.w3[ks], f.h2[ks], y);
@@ -180,7 +184,7 @@ __global__ __launch_bounds__(kMlpThreads) void mlp_forward_kernel(
         float xs[4];
         const bf16x4 xb = load_x(feats, nullptr, sample, valid, mean, stdv, lane, xs);
         Forward f;
-        forward(P, w, xb, lane, f);
+        forward(w, xb, f);
         if (valid && lane < 16) {
             out[static_cast<size_t>(sample) * 3 + 0] = f.y[0];
             out[static_cast<size_t>(sample) * 3 + 1] = f.y[1];
@@ -268,7 +272,7 @@ __global__ __launch_bounds__(kMlpThreads) void mlp_grad_kernel(
         float xs[4];
         const bf16x4 xb = load_x(feats, index, sample, valid, mean, stdv, lane, xs);
         Forward f;
-        forward(P, w, xb, lane, f);
+        forward(w, xb, f);
         // dL/dy for outputs o = 4*(lane/16)+v < 3 of sample lane % 16
         float dy[4];
         const int s = index != nullptr && valid ? index[sample] : sample;
@@ -502,6 +506,12 @@ struct NBXMlp {
 
 namespace nbx {
 
+// Walk record (kWalkRecord floats per tree position, G not applied):
+//   [0..2] total sum, [3] accepted cells, [4..6] accepted-cell sum,
+//   [7] exact leaf terms, [8..13] W xx yy zz xy xz yz with
+//   W = sum over accepted cells of m edge^2 / r^6 d d^T, [14] sum and
+//   [15] minimum of (distance / cell edge)^2 over accepted cells.
+
 // Local frame of one query: e1 along the walk's total acceleration, e2 along
 // the accepted-cell part perpendicular to e1, e3 = e1 x e2. Features are
 // frame-free and the model's outputs are residual components in this frame.
@@ -525,7 +535,7 @@ __device__ inline void local_frame(const float* __restrict__ wf, Frame& fr) {
         fr.e1[1] = 0.f;
         fr.e1[2] = 0.f;
     }
-    const float cx = wf[3], cy = wf[4], cz = wf[5];
+    const float cx = wf[4], cy = wf[5], cz = wf[6];
     const float cp = cx * fr.e1[0] + cy * fr.e1[1] + cz * fr.e1[2];
     float px = cx - cp * fr.e1[0], py = cy - cp * fr.e1[1], pz = cz - cp * fr.e1[2];
     float pn = sqrtf(px * px + py * py + pz * pz);
@@ -534,14 +544,13 @@ __device__ inline void local_frame(const float* __restrict__ wf, Frame& fr) {
     if (!(pn > 1e-6f * a)) {
         // no usable perpendicular cell force: the axis least aligned with e1
         const float ux = fabsf(fr.e1[0]), uy = fabsf(fr.e1[1]), uz = fabsf(fr.e1[2]);
-        const int k = ux <= uy && ux <= uz ? 0 : (uy <= uz ? 1 : 2);
-        const float d = fr.e1[k];
-        px = -d * fr.e1[0];
-        py = -d * fr.e1[1];
-        pz = -d * fr.e1[2];
-        if (k == 0) px += 1.f;
-        if (k == 1) py += 1.f;
-        if (k == 2) pz += 1.f;
+        const bool kx = ux <= uy && ux <= uz;
+        const bool ky = !kx && uy <= uz;
+        const bool kz = !kx && !ky;
+        const float d = kx ? fr.e1[0] : (ky ? fr.e1[1] : fr.e1[2]);
+        px = (kx ? 1.f : 0.f) - d * fr.e1[0];
+        py = (ky ? 1.f : 0.f) - d * fr.e1[1];
+        pz = (kz ? 1.f : 0.f) - d * fr.e1[2];
     }
     // second Gram-Schmidt pass: the first loses orthogonality in FP32 when
     // the perpendicular part is a small difference of large vectors
@@ -558,23 +567,67 @@ __device__ inline void local_frame(const float* __restrict__ wf, Frame& fr) {
     fr.e3[2] = fr.e1[0] * fr.e2[1] - fr.e1[1] * fr.e2[0];
 }
 
-// Raw features from one walk record (12 floats per tree position, G not
-// applied): total sum xyz, accepted-cell sum xyz, accepted cells, exact leaf
-// terms, mean and minimum accepted distance / cell size, sum of m size^2/d^4.
+// Smallest and largest eigenvalue of a symmetric 3x3 matrix (closed form,
+// O. K. Smith, Comm. ACM 4(4):168, 1961).
+__device__ inline void sym3_eig_min_max(float xx, float yy, float zz, float xy, float xz, float yz,
+                                        float& lmin, float& lmax) {
+    const float p1 = xy * xy + xz * xz + yz * yz;
+    const float q = (xx + yy + zz) / 3.f;
+    if (!(p1 > 0.f)) {
+        lmin = fminf(xx, fminf(yy, zz));
+        lmax = fmaxf(xx, fmaxf(yy, zz));
+        return;
+    }
+    const float ax = xx - q, ay = yy - q, az = zz - q;
+    const float p2 = ax * ax + ay * ay + az * az + 2.f * p1;
+    const float p = sqrtf(p2 / 6.f);
+    if (!(p > 0.f)) {
+        lmin = lmax = q;
+        return;
+    }
+    const float ip = 1.f / p;
+    const float bxx = ax * ip, byy = ay * ip, bzz = az * ip;
+    const float bxy = xy * ip, bxz = xz * ip, byz = yz * ip;
+    const float det = bxx * (byy * bzz - byz * byz) - bxy * (bxy * bzz - byz * bxz) +
+                      bxz * (bxy * byz - byy * bxz);
+    const float r = fminf(1.f, fmaxf(-1.f, 0.5f * det));
+    const float phi = acosf(r) / 3.f;
+    lmax = q + 2.f * p * cosf(phi);
+    lmin = q + 2.f * p * cosf(phi + 2.0943951f);
+}
+
+// Raw features (before standardisation) from one walk record.
 __device__ inline void raw_features(const float* __restrict__ wf, const Frame& fr, float m,
                                     int leaf_depth, float log2_root_edge, float f[kF]) {
+    const float cells = wf[3], leaves = wf[7];
     const float inv = fr.a > 0.f ? 1.f / fr.a : 0.f;
-    f[0] = fr.c_par * inv;
-    f[1] = fr.c_perp * inv;
-    f[2] = log2f(1.f + wf[6]);
-    f[3] = log2f(1.f + wf[7]);
-    f[4] = wf[8];
-    f[5] = wf[9];
-    f[6] = log2f(wf[10] * inv + 0x1p-40f);
+    const float wxx = wf[8], wyy = wf[9], wzz = wf[10], wxy = wf[11], wxz = wf[12], wyz = wf[13];
+    const float tr = wxx + wyy + wzz;
+    const float itr = tr > 0.f ? 1.f / tr : 0.f;
+    // e^T W e for the frame axes
+    const float a0 = fr.e1[0], a1 = fr.e1[1], a2 = fr.e1[2];
+    const float b0 = fr.e2[0], b1 = fr.e2[1], b2 = fr.e2[2];
+    const float w1 = wxx * a0 * a0 + wyy * a1 * a1 + wzz * a2 * a2 +
+                     2.f * (wxy * a0 * a1 + wxz * a0 * a2 + wyz * a1 * a2);
+    const float w2 = wxx * b0 * b0 + wyy * b1 * b1 + wzz * b2 * b2 +
+                     2.f * (wxy * b0 * b1 + wxz * b0 * b2 + wyz * b1 * b2);
+    float lmin = 0.f, lmax = 0.f;
+    sym3_eig_min_max(wxx, wyy, wzz, wxy, wxz, wyz, lmin, lmax);
+    f[0] = cells > 0.f ? sqrtf(wf[14] / cells) : 0.f;   // rms distance / cell edge
+    f[1] = cells > 0.f ? sqrtf(wf[15]) : 0.f;           // minimum distance / cell edge
+    f[2] = log2f(tr * inv + 0x1p-40f);                  // truncation scale / |a|
     // own mass over the leaf cell volume: edge = 2 h_root 2^-depth
-    f[7] = log2f(m > 0.f ? m : 1e-30f) - 3.f * (log2_root_edge - static_cast<float>(leaf_depth));
+    f[3] = log2f(m > 0.f ? m : 1e-30f) - 3.f * (log2_root_edge - static_cast<float>(leaf_depth));
+    f[4] = w1 * itr;
+    f[5] = w2 * itr;
+    f[6] = lmin * itr;
+    f[7] = lmax * itr;
+    f[8] = log2f(1.f + cells);
+    f[9] = log2f(1.f + leaves);
+    f[10] = fr.c_par * inv;
+    f[11] = fr.c_perp * inv;
 #pragma unroll
-    for (int k = 8; k < kF; ++k) {
+    for (int k = 12; k < kF; ++k) {
         f[k] = 0.f;
     }
 }
@@ -583,18 +636,24 @@ __global__ void features_kernel(const float* __restrict__ walkf, const float4* _
                                 const int32_t* __restrict__ ndepth,
                                 const uint32_t* __restrict__ scan, const uint32_t* __restrict__ cnt,
                                 const uint32_t* __restrict__ tperm, int n, float log2_root_edge,
-                                float* __restrict__ out, float* __restrict__ frames) {
+                                float G, double* __restrict__ ax, double* __restrict__ ay,
+                                double* __restrict__ az, float* __restrict__ out,
+                                float* __restrict__ frames) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) {
         return;
     }
     float f[kF];
     const int li = static_cast<int>(scan[t] + cnt[t]) + t;
-    const float* wf = walkf + static_cast<size_t>(t) * 12;
+    const float* wf = walkf + static_cast<size_t>(t) * kWalkRecord;
     Frame fr;
     local_frame(wf, fr);
     raw_features(wf, fr, s4[t].w, ndepth[li], log2_root_edge, f);
     const size_t p = tperm[t];
+    // the plain walk's accelerations, bit for bit
+    ax[p] = static_cast<double>(G * wf[0]);
+    ay[p] = static_cast<double>(G * wf[1]);
+    az[p] = static_cast<double>(G * wf[2]);
     float* o = out + p * kF;
 #pragma unroll
     for (int k = 0; k < kF; ++k) {
@@ -611,64 +670,75 @@ __global__ void features_kernel(const float* __restrict__ walkf, const float4* _
     }
 }
 
-// Corrected accelerations: a += |a| y / out_scale, with a the walk's result.
+// Corrected accelerations: a += |a| (y0 e1 + y1 e2 + y2 e3) / out_scale, with
+// a the walk's result. Each wave takes 64 queries per step: every lane
+// assembles one query's features into LDS, four 16-query MFMA passes run the
+// network, and every lane applies its own query's correction.
 __global__ __launch_bounds__(kMlpThreads) void learned_apply_kernel(
     const float* __restrict__ P, const float* __restrict__ mean, const float* __restrict__ stdv,
     float out_scale, const float* __restrict__ walkf, const float4* __restrict__ s4,
     const int32_t* __restrict__ ndepth, const uint32_t* __restrict__ scan,
     const uint32_t* __restrict__ cnt, const uint32_t* __restrict__ tperm, int n,
-    float log2_root_edge, double* __restrict__ ax, double* __restrict__ ay,
+    float log2_root_edge, float G, double* __restrict__ ax, double* __restrict__ ay,
     double* __restrict__ az) {
-    __shared__ float tile[kWavesPerBlock][16][kF + 1];
+    __shared__ float xs_all[kWavesPerBlock][kWave][kF + 1];
+    __shared__ float ys_all[kWavesPerBlock][kWave][4];
     const int lane = threadIdx.x & (kWave - 1);
     const int wid = threadIdx.x / kWave;
+    float (*xs)[kF + 1] = xs_all[wid];
+    float (*ys)[4] = ys_all[wid];
     const int wave = blockIdx.x * kWavesPerBlock + wid;
     const int waves = gridDim.x * kWavesPerBlock;
     Weights w;
     load_weights(P, lane, w);
-    const int groups = (n + 15) / 16;
-    for (int g = wave; g < groups; g += waves) {
-        // lanes 0..15 assemble the raw features of the group's 16 queries
-        if (lane < 16) {
-            const int t = 16 * g + lane;
-            float f[kF];
-            if (t < n) {
-                const int li = static_cast<int>(scan[t] + cnt[t]) + t;
-                const float* wf = walkf + static_cast<size_t>(t) * 12;
-                Frame fr;
-                local_frame(wf, fr);
-                raw_features(wf, fr, s4[t].w, ndepth[li], log2_root_edge, f);
-            } else {
+    const int tiles = (n + kWave - 1) / kWave;
+    for (int g = wave; g < tiles; g += waves) {
+        const int t = kWave * g + lane;
+        const bool valid = t < n;
+        Frame fr;
+        float f[kF];
+        double bx = 0.0, by = 0.0, bz = 0.0;
+        if (valid) {
+            const int li = static_cast<int>(scan[t] + cnt[t]) + t;
+            const float* wf = walkf + static_cast<size_t>(t) * kWalkRecord;
+            local_frame(wf, fr);
+            raw_features(wf, fr, s4[t].w, ndepth[li], log2_root_edge, f);
+            // the plain walk's accelerations, bit for bit
+            bx = static_cast<double>(G * wf[0]);
+            by = static_cast<double>(G * wf[1]);
+            bz = static_cast<double>(G * wf[2]);
+        } else {
 #pragma unroll
-                for (int k = 0; k < kF; ++k) f[k] = 0.f;
-            }
+            for (int k = 0; k < kF; ++k) f[k] = mean[k];
+        }
 #pragma unroll
-            for (int k = 0; k < kF; ++k) tile[wid][lane][k] = f[k];
+        for (int k = 0; k < kF; ++k) {
+            xs[lane][k] = (f[k] - mean[k]) / stdv[k];
         }
         wave_lds_sync();
-        const int p = lane & 15;
         const int f0 = 4 * (lane >> 4);
-        const bf16x4 xb = pack((tile[wid][p][f0] - mean[f0]) / stdv[f0],
-                               (tile[wid][p][f0 + 1] - mean[f0 + 1]) / stdv[f0 + 1],
-                               (tile[wid][p][f0 + 2] - mean[f0 + 2]) / stdv[f0 + 2],
-                               (tile[wid][p][f0 + 3] - mean[f0 + 3]) / stdv[f0 + 3]);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int p = 16 * q + (lane & 15);
+            const bf16x4 xb = pack(xs[p][f0], xs[p][f0 + 1], xs[p][f0 + 2], xs[p][f0 + 3]);
+            Forward fw;
+            forward(w, xb, fw);
+            if (lane < 16) {
+                ys[p][0] = fw.y[0];
+                ys[p][1] = fw.y[1];
+                ys[p][2] = fw.y[2];
+            }
+        }
         wave_lds_sync();
-        Forward fw;
-        forward(P, w, xb, lane, fw);
-        const int t = 16 * g + p;
-        if (lane < 16 && t < n) {
-            // the walk stored G * (cells + leaves) in FP64 for this particle;
-            // the outputs are the residual / |a| in the query's local frame
-            Frame fr;
-            local_frame(walkf + static_cast<size_t>(t) * 12, fr);
+        if (valid) {
+            const float y0 = ys[lane][0], y1 = ys[lane][1], y2 = ys[lane][2];
             const uint32_t o = tperm[t];
-            const double bx = ax[o], by = ay[o], bz = az[o];
             const double k = sqrt(bx * bx + by * by + bz * bz) / out_scale;
-            const float y0 = fw.y[0], y1 = fw.y[1], y2 = fw.y[2];
             ax[o] = bx + k * (y0 * fr.e1[0] + y1 * fr.e2[0] + y2 * fr.e3[0]);
             ay[o] = by + k * (y0 * fr.e1[1] + y1 * fr.e2[1] + y2 * fr.e3[1]);
             az[o] = bz + k * (y0 * fr.e1[2] + y1 * fr.e2[2] + y2 * fr.e3[2]);
         }
+        wave_lds_sync();
     }
 }
 
@@ -679,18 +749,19 @@ NBodySimProStatus bh_learned(Engine* e, const NBodySimProGravity* g, NBodySimPro
     }
     NBX_TRY(bh_build(e, true, error));
     BHState& s = e->bh;
-    NBX_TRY(e->scratch2.need<float>(s.n * 12, error));
+    NBX_TRY(e->scratch2.need<float>(s.n * kWalkRecord, error));
     float* walkf = e->scratch2.as<float>();
     NBX_TRY(bh_walk(e, g, false, walkf, error));
     NBX_HIP(error, hipEventRecord(e->ev[7], e->stream));
-    const int groups = static_cast<int>((s.n + 15) / 16);
-    int blocks = (groups + kWavesPerBlock - 1) / kWavesPerBlock;
-    blocks = blocks < 256 * 8 ? blocks : 256 * 8;
+    const int tiles = static_cast<int>((s.n + kWave - 1) / kWave);
+    int blocks = (tiles + kWavesPerBlock - 1) / kWavesPerBlock;
+    blocks = blocks < m->cu_count * 4 ? blocks : m->cu_count * 4;
     learned_apply_kernel<<<blocks, kMlpThreads, 0, e->stream>>>(
         m->params, m->norm, m->norm + kF, m->out_scale, walkf, s.s4.as<float4>(),
         s.ndepth.as<int32_t>(), s.scan.as<uint32_t>(), s.cnt.as<uint32_t>(),
         s.tperm.as<uint32_t>(), static_cast<int>(s.n),
-        static_cast<float>(std::log2(2.0 * s.root[3])), e->ax, e->ay, e->az);
+        static_cast<float>(std::log2(2.0 * s.root[3])),
+        static_cast<float>(g->gravitational_constant), e->ax, e->ay, e->az);
     NBX_HIP(error, hipGetLastError());
     NBX_HIP(error, hipEventRecord(e->ev[8], e->stream));
     NBX_HIP(error, hipEventSynchronize(e->ev[8]));
@@ -979,7 +1050,7 @@ NBodySimProStatus nbx_engine_bh_features(NBXEngine* engine, const NBodySimProGra
     NBX_HIP(error, hipSetDevice(e->device));
     NBX_TRY(nbx::bh_build(e, true, error));
     nbx::BHState& s = e->bh;
-    NBX_TRY(e->scratch2.need<float>(s.n * 12, error));
+    NBX_TRY(e->scratch2.need<float>(s.n * nbx::kWalkRecord, error));
     NBX_TRY(e->scratch.need<float>(s.n * (nbx::kF + 9), error));
     NBX_TRY(nbx::bh_walk(e, g, false, e->scratch2.as<float>(), error));
     float* dfeat = e->scratch.as<float>();
@@ -987,7 +1058,9 @@ NBodySimProStatus nbx_engine_bh_features(NBXEngine* engine, const NBodySimProGra
     nbx::features_kernel<<<nbx::grid_for(s.n, 256), 256, 0, e->stream>>>(
         e->scratch2.as<float>(), s.s4.as<float4>(), s.ndepth.as<int32_t>(), s.scan.as<uint32_t>(),
         s.cnt.as<uint32_t>(), s.tperm.as<uint32_t>(), static_cast<int>(s.n),
-        static_cast<float>(std::log2(2.0 * s.root[3])), dfeat, frames != nullptr ? dframe : nullptr);
+        static_cast<float>(std::log2(2.0 * s.root[3])),
+        static_cast<float>(g->gravitational_constant), e->ax, e->ay, e->az, dfeat,
+        frames != nullptr ? dframe : nullptr);
     NBX_HIP(error, hipGetLastError());
     NBX_HIP(error, hipMemcpyAsync(features, dfeat, s.n * nbx::kF * sizeof(float),
                                   hipMemcpyDeviceToHost, e->stream));

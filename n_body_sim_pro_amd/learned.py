@@ -25,7 +25,9 @@ from .engine import (Engine, NBXError, _check, _learned_lib, disk_galaxy, plumme
 
 N_FEATURES = 16
 N_HIDDEN = 64
-N_USED_FEATURES = 8
+# Inputs the model uses (include/nbx_learned.h numbering); the others get
+# std = +inf and standardise to 0.
+FEATURES = (0, 1, 2, 3, 4, 5, 6, 7)
 WEIGHTS = pathlib.Path(__file__).resolve().parent / "weights" / "bh_learned_theta1.npz"
 _F = C.POINTER(C.c_float)
 
@@ -140,12 +142,13 @@ def probe_mfma(a, b, device=0):
     return d
 
 
-def init_params(seed):
-    """He-style initial weights in the nbx_learned.h layout; padding rows zero."""
+def init_params(seed, features=FEATURES):
+    """He-style initial weights in the nbx_learned.h layout; the columns of
+    unused features and the padding rows are zero."""
     rng = np.random.default_rng(seed)
     F, H = N_FEATURES, N_HIDDEN
-    W1 = rng.normal(0.0, math.sqrt(2.0 / N_USED_FEATURES), (H, F))
-    W1[:, N_USED_FEATURES:] = 0.0
+    W1 = np.zeros((H, F))
+    W1[:, list(features)] = rng.normal(0.0, math.sqrt(2.0 / len(features)), (H, len(features)))
     W2 = rng.normal(0.0, math.sqrt(2.0 / H), (H, H))
     W3 = np.zeros((16, H))
     W3[:3] = rng.normal(0.0, math.sqrt(1.0 / H), (3, H)) * 0.1
@@ -188,28 +191,31 @@ def training_set(n=65536, seeds=(1, 2, 3, 4), theta=1.0, G=1.0, eps=0.01, device
     return np.concatenate(X), np.concatenate(Y)
 
 
-def fit_scaling(feats, resid, clip=8.0):
+def fit_scaling(feats, resid, features=FEATURES, clip=8.0):
     """Feature standardisation, robust output scale and clipped scaled targets."""
     mean = feats.mean(axis=0).astype(np.float32)
     std = feats.std(axis=0).astype(np.float32)
-    unused = std < 1e-6
+    unused = np.ones(N_FEATURES, bool)
+    unused[list(features)] = False
+    unused |= std < 1e-6
     mean[unused] = 0.0
-    std[unused] = 1.0
+    std[unused] = np.inf
     mad = np.median(np.abs(resid))
     out_scale = float(1.0 / (1.4826 * mad)) if mad > 0 else 1.0
     targets = np.clip(resid * out_scale, -clip, clip).astype(np.float32)
     return mean, std, out_scale, targets
 
 
-def train(feats, resid, epochs=60, batch=1024, lr=2e-3, seed=0, device=0, log=None):
+def train(feats, resid, epochs=60, batch=1024, lr=2e-3, seed=0, device=0, features=FEATURES,
+          log=None):
     """Trains a model on the device; returns it and a report."""
-    mean, std, out_scale, targets = fit_scaling(feats, resid)
+    mean, std, out_scale, targets = fit_scaling(feats, resid, features)
     rng = np.random.default_rng(seed)
     order = rng.permutation(len(feats))
     n_val = max(len(feats) // 10, 1)
     val, tr = order[:n_val], order[n_val:]
     mlp = Mlp(device)
-    mlp.set_params(init_params(seed), mean, std, out_scale)
+    mlp.set_params(init_params(seed, features), mean, std, out_scale)
     mlp.load_dataset(feats[tr], targets[tr])
     history = []
     t0 = time.time()
@@ -246,7 +252,9 @@ def main(argv=None):
     ap.add_argument("--batch", type=int, default=1024)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--features", default=",".join(map(str, FEATURES)))
     a = ap.parse_args(argv)
+    features = tuple(int(k) for k in a.features.split(","))
 
     def log(msg):
         print(msg, flush=True)
@@ -254,9 +262,10 @@ def main(argv=None):
     seeds = tuple(int(s) for s in a.seeds.split(","))
     X, Y = training_set(a.n, seeds, a.theta, 1.0, a.eps, log=log)
     log(f"training set {X.shape[0]} samples")
-    mlp, rep = train(X, Y, a.epochs, a.batch, a.lr, a.seed, log=log)
+    mlp, rep = train(X, Y, a.epochs, a.batch, a.lr, a.seed, features=features, log=log)
     meta = {"theta": a.theta, "eps": a.eps, "n": a.n, "seeds": list(seeds),
             "epochs": a.epochs, "batch": a.batch, "lr": a.lr, "seed": a.seed,
+            "features": list(features),
             "val_loss": rep["history"][-1][2] if rep["history"] else None,
             "val_loss_zero_model": rep["val_loss_zero_model"]}
     mlp.save(a.out, **meta)
