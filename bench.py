@@ -53,6 +53,12 @@ def parse():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ens-clusters", type=int, default=4096)
     ap.add_argument("--ens-steps", type=int, default=50)
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--learned-n", type=int, default=10_000_000)
+    ap.add_argument("--learned-evals", type=int, default=3)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run a plain elementwise copy of this many MiB (3 launches): "
+                         "the known-size kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE")
     return ap.parse_args()
 
 
@@ -175,8 +181,82 @@ def ensemble(nbx, clusters, steps, eps, dt, seed):
             "drift_interval_steps": 2 * steps, "autotune": tune}
 
 
+def learned_bh(nbx, n, seed, eps, evals, sample=8192):
+    """Learned correction at N = n two-galaxy: theta = 1.0 walk + the committed
+    MLP (n_body_sim_pro_amd/weights) against plain theta = 1.0, 0.7 and 0.5,
+    errors from the FP64 direct sum (reference scalar order) on a seeded
+    `sample`-particle subsample."""
+    from n_body_sim_pro_amd import learned as L
+    t0 = time.perf_counter()
+    d = nbx.two_galaxy(n, seed=seed)
+    t_gen = time.perf_counter() - t0
+    idx = np.sort(np.random.default_rng(7).choice(n, size=min(sample, n), replace=False))
+    out = {"n": n, "sample": int(idx.size), "generation_s": t_gen}
+    mlp = L.Mlp.load(L.WEIGHTS)
+    with nbx.Engine(n, theta=1.0) as e:
+        e.upload(d["pos"], d["m"])
+        t0 = time.perf_counter()
+        ref = e.direct_fp64_targets(idx, 1.0, eps)
+        out["reference_fp64_subsample_s"] = time.perf_counter() - t0
+        rn = np.linalg.norm(ref, axis=1)
+        for name, theta, algo, model in (("bh_theta1.0", 1.0, "hip_barnes_hut", None),
+                                          ("learned_theta1.0", 1.0, "hip_bh_learned", mlp),
+                                          ("bh_theta0.7", 0.7, "hip_barnes_hut", None),
+                                          ("bh_theta0.5", 0.5, "hip_barnes_hut", None)):
+            e.theta = theta
+            e.set_mlp(model)
+            e.compute_force(algo, 1.0, eps)   # warm-up
+            ts = []
+            for _ in range(evals):
+                e.compute_force(algo, 1.0, eps)
+                ts.append(e.timings())
+            err = np.linalg.norm(e.accelerations()[idx] - ref, axis=1) / rn
+            ms = float(np.mean([t["force_ms"] for t in ts]))
+            out[name] = {"ms": ms, "mparticles_s": n / (ms * 1e-3) / 1e6,
+                         "walk_ms": float(np.mean([t["walk_ms"] for t in ts])),
+                         "median_rel_err": float(np.median(err)),
+                         "p90_rel_err": float(np.quantile(err, 0.9)),
+                         "max_rel_err": float(err.max())}
+            if model is not None:
+                out[name]["mlp_ms"] = float(np.mean([t["integrate_ms"] for t in ts]))
+            e.set_mlp(None)
+    mlp.close()
+    lrn, b5 = out["learned_theta1.0"], out["bh_theta0.5"]
+    out["speedup_vs_theta0.5"] = b5["ms"] / lrn["ms"]
+    out["error_ratio_vs_theta0.5"] = lrn["median_rel_err"] / b5["median_rel_err"]
+    out["error_gain_vs_theta1.0"] = out["bh_theta1.0"]["median_rel_err"] / lrn["median_rel_err"]
+    out["target_met"] = bool(lrn["median_rel_err"] <= b5["median_rel_err"]
+                             and out["speedup_vs_theta0.5"] >= 2.0)
+    return out
+
+
+def measured_traffic(name):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 PMC
+    summary (profiles/<name>_traffic.json), if one was recorded for this
+    workload; None otherwise."""
+    path = ROOT / "profiles" / f"{name}_traffic.json"
+    if not path.exists():
+        return None
+    try:
+        return json.loads(path.read_text())
+    except (OSError, ValueError):
+        return None
+
+
 def main():
     args = parse()
+    if args.copy_probe_mib > 0:
+        import torch
+        n = args.copy_probe_mib * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            torch.add(a, 1.0, out=b)   # reads 4n bytes, writes 4n bytes
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_mib": args.copy_probe_mib, "read_bytes": 4 * n,
+                          "write_bytes": 4 * n, "launches": 3}))
+        return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
               "(multi-GPU sharding is described, not implemented).", file=sys.stderr)
@@ -237,17 +317,24 @@ def main():
                "phase_ms_last": {k: tm[k] for k in ("bbox_keys_ms", "sort_ms", "tree_ms",
                                                     "com_ms", "walk_ms")},
                "generation_s": t_gen},
-        "roofline": {"kernel": "walk_kernel<false> (FP32 wave-coherent walk)",
+        "roofline": {"kernel": "walk_f32_kernel<false> (FP32 wave-coherent walk)",
                      "bound": "mfma", "pipe": "valu", "achieved": achieved,
                      "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
-                     "flops_per_unit": "20 per interaction (approximations + exact)"},
+                     "flops_per_unit": "20 per interaction (approximations + exact)",
+                     "launch_ms": walk},
     }
+    tr = measured_traffic("walk_f32_two_galaxy_1M")
+    if tr is not None:
+        out["roofline"]["traffic"] = tr.get("bytes_per_launch")
+        out["roofline"]["traffic_source"] = tr.get("source")
     if not args.no_all_pairs:
         out["all_pairs"] = all_pairs(nbx, args.ap_n, args.ap_evals, args.eps, args.seed)
     if not args.no_ensemble:
         out["ensemble"] = ensemble(nbx, args.ens_clusters, args.ens_steps, args.eps, args.dt,
                                    args.seed)
+    if not args.no_learned:
+        out["learned"] = learned_bh(nbx, args.learned_n, args.seed, args.eps, args.learned_evals)
     if not args.no_cpu_baseline:
         pos32 = d["pos"].astype(np.float32).astype(np.float64)
         m32 = d["m"].astype(np.float32).astype(np.float64)
