@@ -48,6 +48,7 @@ def parse():
     ap.add_argument("--seed", type=int, default=42)
     ap.add_argument("--ap-n", type=int, default=262_144)
     ap.add_argument("--ap-evals", type=int, default=5)
+    ap.add_argument("--ap-drift-steps", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-all-pairs", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
@@ -126,7 +127,7 @@ def cpu_baseline_bh(pos, m, theta, eps, evals=2):
             "interactions": int(stats.accepted_approximations + stats.exact_interactions)}, acc
 
 
-def all_pairs(nbx, n, evals, eps, seed):
+def all_pairs(nbx, n, evals, eps, seed, drift_steps, dt):
     d = nbx.plummer(n, seed=seed)
     with nbx.Engine(n) as e:
         e.upload(d["pos"], d["m"], d["vel"])
@@ -138,6 +139,18 @@ def all_pairs(nbx, n, evals, eps, seed):
             e.compute_force("hip_direct_fp32", eps=eps)
             ks.append(e.timings())
         acc = e.accelerations()
+        drift = None
+        if drift_steps > 0:
+            # FP32 pair forces, FP64 KDK state and FP64 energies (O(N^2) potential)
+            q0 = e.diagnostics(eps=eps)
+            e.advance("hip_direct_fp32", dt=dt, steps=drift_steps, eps=eps)
+            q1 = e.diagnostics(eps=eps)
+            e0 = q0["kinetic_energy"] + q0["potential_energy"]
+            e1 = q1["kinetic_energy"] + q1["potential_energy"]
+            p0 = np.array([q0[f"total_momentum_{c}"] for c in "xyz"])
+            p1 = np.array([q1[f"total_momentum_{c}"] for c in "xyz"])
+            drift = {"steps": drift_steps, "dt": dt, "relative_energy_drift": (e1 - e0) / abs(e0),
+                     "momentum_change": float(np.linalg.norm(p1 - p0))}
     idx = np.random.default_rng(7).choice(n, min(4096, n), replace=False)
     num = np.linalg.norm(acc[idx] - ref[idx], axis=1)
     den = np.linalg.norm(ref[idx], axis=1)
@@ -151,7 +164,7 @@ def all_pairs(nbx, n, evals, eps, seed):
             "median_rel_err": float(np.median(rel)), "max_rel_err": float(rel.max()),
             "error_sample": f"{idx.size} seeded targets vs the FP64 direct sum "
                             "(hip_direct_fp64, bit-identical to the reference scalar kernel)",
-            "dtype": "f32 pairs, compensated f32 sums"}
+            "dtype": "f32 pairs, compensated f32 sums", "energy_drift": drift}
 
 
 def ensemble(nbx, clusters, steps, eps, dt, seed):
@@ -329,7 +342,8 @@ def main():
         out["roofline"]["traffic"] = tr.get("bytes_per_launch")
         out["roofline"]["traffic_source"] = tr.get("source")
     if not args.no_all_pairs:
-        out["all_pairs"] = all_pairs(nbx, args.ap_n, args.ap_evals, args.eps, args.seed)
+        out["all_pairs"] = all_pairs(nbx, args.ap_n, args.ap_evals, args.eps, args.seed,
+                                     args.ap_drift_steps, args.dt)
     if not args.no_ensemble:
         out["ensemble"] = ensemble(nbx, args.ens_clusters, args.ens_steps, args.eps, args.dt,
                                    args.seed)

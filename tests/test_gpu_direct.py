@@ -47,6 +47,51 @@ def test_direct_fp64_plummer_16384_bit_identical(nbx, oracle):
     assert rel_err_per_particle(acc, q.acc()).max() <= 1e-12
 
 
+def test_direct_fp64_plummer_16384_three_kdk_steps_bit_identical(nbx, oracle):
+    """Config 1: N=16384 Plummer, eps=0.01, leapfrog dt=1e-3, 3 steps (4 force
+    evaluations incl. the initial one) against the oracle's KDK with the
+    reference's scalar force: positions, velocities and accelerations agree
+    bit for bit (so within the 1e-12 relative bound)."""
+    p = from_gen(nbx.plummer(16384, seed=42))
+    g = gravity(1.0, 0.01)
+    with nbx.Engine(16384) as e:
+        e.upload_view(p.view)
+        e.compute_force("hip_direct_fp64", eps=0.01)
+        e.advance("hip_direct_fp64", dt=1e-3, steps=3, eps=0.01)
+        s = e.download()
+    q = p.copy()
+    fn = C.cast(oracle.nbo_direct_openmp, C.c_void_p)
+    oracle.nbo_direct_openmp(C.byref(q.view), C.byref(g))
+    for _ in range(3):
+        assert oracle.nbo_leapfrog_advance(C.byref(q.view), C.byref(g), 1e-3, fn) == 0
+    assert np.array_equal(s["pos"], q.pos())
+    assert np.array_equal(s["vel"], np.stack([q["vx"], q["vy"], q["vz"]], 1))
+    assert np.array_equal(s["acc"], q.acc())
+    assert rel_err_per_particle(s["acc"], q.acc()).max() <= 1e-12
+
+
+def test_direct_fp32_energy_drift_100_steps(nbx):
+    """Config 3 shape at N=16384: 100 KDK steps (dt=1e-3, eps=0.01) with FP32
+    pair forces and the FP64 integrator; the relative energy drift matches the
+    drift of the same run with FP64 forces (hip_direct_fp64, bit-identical to
+    the reference kernel) to 1e-6, and stays below 1e-4."""
+    d = nbx.plummer(16384, seed=3)
+    drift = {}
+    for algo in ("hip_direct_fp32", "hip_direct_fp64"):
+        with nbx.Engine(16384) as e:
+            e.upload(d["pos"], d["m"], d["vel"])
+            e.compute_force(algo, eps=0.01)
+            q0 = e.diagnostics(eps=0.01)
+            e.advance(algo, dt=1e-3, steps=100, eps=0.01)
+            q1 = e.diagnostics(eps=0.01)
+        e0 = q0["kinetic_energy"] + q0["potential_energy"]
+        e1 = q1["kinetic_energy"] + q1["potential_energy"]
+        drift[algo] = (e1 - e0) / abs(e0)
+    print("relative energy drift over 100 steps", drift)
+    assert abs(drift["hip_direct_fp32"] - drift["hip_direct_fp64"]) < 1e-6
+    assert abs(drift["hip_direct_fp32"]) < 1e-4
+
+
 @pytest.mark.parametrize("n", [1, 2, 3, 63, 64, 65, 257])
 def test_direct_fp64_ragged_sizes_and_zero_softening(nbx, oracle, n):
     rng = np.random.default_rng(n)

@@ -81,7 +81,7 @@ def _loss64(p64, mean, std, feats, targets):
     P = {"W1": p64[mo.OFF_W1:mo.OFF_B1].reshape(mo.H, mo.F), "b1": p64[mo.OFF_B1:mo.OFF_W2],
          "W2": p64[mo.OFF_W2:mo.OFF_B2].reshape(mo.H, mo.H), "b2": p64[mo.OFF_B2:mo.OFF_W3],
          "W3": p64[mo.OFF_W3:mo.OFF_B3].reshape(16, mo.H), "b3": p64[mo.OFF_B3:]}
-    x = ((feats.astype(np.float64) - mean) / std).T
+    x = ((feats.astype(np.float64) - mean) * (np.float32(1) / std.astype(np.float32))).T
     silu = lambda z: z / (1 + np.exp(-z))  # noqa: E731
     h1 = silu(P["W1"] @ x + P["b1"][:, None])
     h2 = silu(P["W2"] @ h1 + P["b2"][:, None])
