@@ -26,9 +26,11 @@
 // pointers visits, for every query, the same nodes in the same order.
 //
 // Centre of mass. Computed in FP64 from the children in octant order 0..7
-// (barnes_hut.c:230-244), one thread per node chosen by an arrival counter
-// (the last child to finish computes its parent), so results are
-// deterministic and bit-identical to the reference.
+// (barnes_hut.c:230-244), one thread per internal node, deepest level first:
+// the nodes are bucketed by depth during the fill, each large level is one
+// launch and the small levels near the root run in a single one-block launch.
+// A node reads its children through an 8-slot child table written by the
+// fill, so results are deterministic and bit-identical to the reference.
 //
 // Walk. One wave = 64 consecutive queries in tree order; the node index is
 // wave-uniform (node records come in through scalar loads). Each lane keeps
