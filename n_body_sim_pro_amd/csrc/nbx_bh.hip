@@ -555,16 +555,17 @@ __global__ __launch_bounds__(kThreads) void fill_kernel(
     nd.cz[li] = z;
     nd.m[li] = m;
     nd.minpos[li] = t;
-    const double h = ldexp(root_h, -depth);
-    NodeF64 a;
-    a.cx = x;
-    a.cy = y;
-    a.cz = z;
-    a.m = m;
-    a.h = h;
-    a.skip = li + 1;
-    a.leaf = t;
-    n64[li] = a;
+    if (n64 != nullptr) {   // FP64 records only for the exact FP64 walk
+        NodeF64 a;
+        a.cx = x;
+        a.cy = y;
+        a.cz = z;
+        a.m = m;
+        a.h = ldexp(root_h, -depth);
+        a.skip = li + 1;
+        a.leaf = t;
+        n64[li] = a;
+    }
     NodeF32 b;
     b.cx = static_cast<float>(x);
     b.cy = static_cast<float>(y);
@@ -639,15 +640,17 @@ __device__ inline void com_node(int iid, const int* __restrict__ inode,
     nd.minpos[p] = minpos;
     const int depth = nd.depth[p];
     const double h = ldexp(root_h, -depth);
-    NodeF64 a;
-    a.cx = ox;
-    a.cy = oy;
-    a.cz = oz;
-    a.m = mass;
-    a.h = h;
-    a.skip = end;
-    a.leaf = -1;
-    n64[p] = a;
+    if (n64 != nullptr) {
+        NodeF64 a;
+        a.cx = ox;
+        a.cy = oy;
+        a.cz = oz;
+        a.m = mass;
+        a.h = h;
+        a.skip = end;
+        a.leaf = -1;
+        n64[p] = a;
+    }
     NodeF32 b;
     b.cx = static_cast<float>(ox);
     b.cy = static_cast<float>(oy);
@@ -774,6 +777,11 @@ __global__ __launch_bounds__(kThreads) void walk_f64_kernel(
 // order): its L2 then holds that eighth's part of the tree. `grid` is a
 // multiple of kXcds; surplus blocks get indices >= n and only walk the root.
 constexpr int kXcds = 8;
+#ifndef NBX_WALK_QUERIES
+#define NBX_WALK_QUERIES 2
+#endif
+// Queries per lane of the FP32 walk (1: walk_f32_kernel, else walk_f32_multi_kernel).
+constexpr int kWalkQueries = NBX_WALK_QUERIES;
 __device__ inline int xcd_block(int b, int grid) {
     return (b % kXcds) * (grid / kXcds) + b / kXcds;
 }
@@ -891,6 +899,102 @@ __global__ __launch_bounds__(kThreads) void walk_f32_kernel(
     }
 }
 
+// Q queries per lane (tree positions base + 64k + lane, k < Q): a wave walks
+// the union of 64 Q consecutive queries' interaction lists, so every node
+// record fetched (the walk's latency chain) feeds Q times the arithmetic.
+// Each query keeps its own skip_until and its reference visit order.
+template <int Q>
+__global__ __launch_bounds__(kThreads) void walk_f32_multi_kernel(
+    const NodeF32* __restrict__ n32, int nodes, const float4* __restrict__ s4,
+    const uint32_t* __restrict__ tperm, int n, float G, float eps2, float theta2,
+    double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az,
+    unsigned long long* __restrict__ counters) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = xcd_block(blockIdx.x, gridDim.x) * (kThreads / kWave) + threadIdx.x / kWave;
+    const int base = wave * (kWave * Q);
+    int t[Q], skip_until[Q];
+    float qx[Q], qy[Q], qz[Q], acx[Q], acy[Q], acz[Q];
+    unsigned approx[Q], exact[Q];
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        t[k] = base + k * kWave + lane;
+        const bool valid = t[k] < n;
+        const float4 q = s4[valid ? t[k] : n - 1];
+        qx[k] = q.x;
+        qy[k] = q.y;
+        qz[k] = q.z;
+        skip_until[k] = valid ? 0 : 0x7FFFFFFF;
+        acx[k] = acy[k] = acz[k] = 0.f;
+        approx[k] = exact[k] = 0u;
+    }
+    int i = 0;
+    while (i < nodes) {
+        const NodeF32 cur = n32[__builtin_amdgcn_readfirstlane(i)];
+        if (cur.leaf >= 0) {
+#pragma unroll
+            for (int k = 0; k < Q; ++k) {
+                if (i >= skip_until[k] && cur.leaf != t[k]) {
+                    const float dx = cur.cx - qx[k];
+                    const float dy = cur.cy - qy[k];
+                    const float dz = cur.cz - qz[k];
+                    const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
+                    const float inv = __builtin_amdgcn_rsqf(r2);
+                    const float s = (cur.m * inv) * (inv * inv);
+                    acx[k] = fmaf(s, dx, acx[k]);
+                    acy[k] = fmaf(s, dy, acy[k]);
+                    acz[k] = fmaf(s, dz, acz[k]);
+                    ++exact[k];
+                }
+            }
+            i = i + 1;
+        } else {
+            bool descend = false;
+#pragma unroll
+            for (int k = 0; k < Q; ++k) {
+                if (i >= skip_until[k]) {
+                    const float dx = cur.cx - qx[k];
+                    const float dy = cur.cy - qy[k];
+                    const float dz = cur.cz - qz[k];
+                    const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                    if (d2 > 0.f && cur.size2 < theta2 * d2) {
+                        const float r2 = d2 + eps2;
+                        const float inv = __builtin_amdgcn_rsqf(r2);
+                        const float s = (cur.m * inv) * (inv * inv);
+                        acx[k] = fmaf(s, dx, acx[k]);
+                        acy[k] = fmaf(s, dy, acy[k]);
+                        acz[k] = fmaf(s, dz, acz[k]);
+                        ++approx[k];
+                        skip_until[k] = cur.skip;
+                    } else {
+                        descend = true;
+                    }
+                }
+            }
+            i = __builtin_amdgcn_ballot_w64(descend) != 0 ? i + 1 : cur.skip;
+        }
+    }
+    unsigned long long wa = 0, we = 0;
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        if (t[k] < n) {
+            const uint32_t o = tperm[t[k]];
+            ax[o] = static_cast<double>(G * acx[k]);
+            ay[o] = static_cast<double>(G * acy[k]);
+            az[o] = static_cast<double>(G * acz[k]);
+            wa += approx[k];
+            we += exact[k];
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        wa += __shfl_xor(wa, off);
+        we += __shfl_xor(we, off);
+    }
+    if (lane == 0) {
+        atomicAdd(&counters[0], wa);
+        atomicAdd(&counters[1], we);
+    }
+}
+
 template <typename T>
 T* B(Buf& b) {
     return b.as<T>();
@@ -995,11 +1099,8 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error)
 
     // Morton order: stable sort of (key, original index) on the 63 key bits.
     NBX_TRY(radix_sort_pairs(st, B<uint64_t>(s.sk0), B<uint32_t>(s.sv0), B<uint64_t>(s.sk1),
-                             B<uint32_t>(s.sv1), n, 63, s.hist, error));
-    NBX_HIP(error, hipMemcpyAsync(s.mkey_sorted.p, s.sk0.p, n * sizeof(uint64_t),
-                                  hipMemcpyDeviceToDevice, st));
-    NBX_HIP(error, hipMemcpyAsync(s.mperm.p, s.sv0.p, n * sizeof(uint32_t),
-                                  hipMemcpyDeviceToDevice, st));
+                             B<uint32_t>(s.sv1), n, 63, s.hist, error,
+                             B<uint64_t>(s.mkey_sorted), B<uint32_t>(s.mperm)));
     unsigned hflags[kDOff];
     NBX_HIP(error, hipMemcpyAsync(hflags, flags, 8 * sizeof(unsigned), hipMemcpyDeviceToHost, st));
     NBX_HIP(error, hipMemcpyAsync(s.root, root, 4 * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -1103,14 +1204,21 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error)
     NBX_TRY(s.ninode.need<int>(internal + 1, error));
     NBX_TRY(s.nkids.need<int>(8 * internal + 8, error));
     NBX_HIP(error, hipMemsetAsync(s.nkids.p, 0xFF, (8 * internal + 8) * sizeof(int), st));
-    NBX_TRY(s.n64.need<NodeF64>(nodes, error));
+    // FP64 node records feed only the exact FP64 walk
+    NodeF64* n64_out = nullptr;
+    s.has_n64 = false;
+    if (!fp32_inputs) {
+        NBX_TRY(s.n64.need<NodeF64>(nodes, error));
+        n64_out = B<NodeF64>(s.n64);
+        s.has_n64 = true;
+    }
     NBX_TRY(s.n32.need<NodeF32>(nodes, error));
     NodeSoA nd = node_soa(s);
     depth_offsets_kernel<<<1, 1, 0, st>>>(flags + kDCount, flags + kDOff);
     fill_kernel<<<g, kThreads, 0, st>>>(B<uint64_t>(s.tword), ext_tree, B<int32_t>(s.delta),
                                         B<uint32_t>(s.cnt), B<uint32_t>(s.scan), ni,
                                         B<double>(s.sx), B<double>(s.sy), B<double>(s.sz),
-                                        B<double>(s.sm), s.root[3], nd, B<NodeF64>(s.n64),
+                                        B<double>(s.sm), s.root[3], nd, n64_out,
                                         B<NodeF32>(s.n32), flags + kDOff, flags + kDCursor,
                                         B<int>(s.nlist), B<int>(s.ninode), B<int>(s.nkids));
     NBX_HIP(error, hipGetLastError());
@@ -1130,7 +1238,7 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error)
                 B<int>(s.nlist) + offs[d], static_cast<int>(count), B<int>(s.ninode),
                 B<int>(s.nkids), nd, B<uint32_t>(s.trank),
                 B<double>(s.sx), B<double>(s.sy), B<double>(s.sz), b4, s.root[3],
-                B<NodeF64>(s.n64), B<NodeF32>(s.n32));
+                n64_out, B<NodeF32>(s.n32));
             --d;
             continue;
         }
@@ -1141,7 +1249,7 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error)
         com_small_levels_kernel<<<1, 1024, 0, st>>>(
             B<int>(s.nlist), flags + kDOff, flags + kDCount, d, lo, B<int>(s.ninode),
             B<int>(s.nkids), nd, B<uint32_t>(s.trank),
-            B<double>(s.sx), B<double>(s.sy), B<double>(s.sz), b4, s.root[3], B<NodeF64>(s.n64),
+            B<double>(s.sx), B<double>(s.sy), B<double>(s.sz), b4, s.root[3], n64_out,
             B<NodeF32>(s.n32));
         d = lo - 1;
     }
@@ -1172,6 +1280,10 @@ NBodySimProStatus bh_walk(Engine* e, const NBodySimProGravity* g, bool fp64_exac
     const double theta2 = e->theta * e->theta;
     const unsigned blocks = grid_for(s.n, kThreads);
     const unsigned grid = (blocks + kXcds - 1) / kXcds * kXcds;
+    if (fp64_exact && !s.has_n64) {
+        NBX_FAIL(error, N_BODY_SIM_PRO_STATUS_INVALID_STATE,
+                 "the exact walk needs a build on FP64 inputs");
+    }
     if (fp64_exact) {
         walk_f64_kernel<<<blocks, kThreads, 0, st>>>(
             B<NodeF64>(s.n64), static_cast<int>(s.nodes), B<double>(s.sx), B<double>(s.sy),
@@ -1183,12 +1295,20 @@ NBodySimProStatus bh_walk(Engine* e, const NBodySimProGravity* g, bool fp64_exac
             ni, static_cast<float>(g->gravitational_constant),
             static_cast<float>(g->softening_squared), static_cast<float>(theta2), e->ax, e->ay,
             e->az, B<unsigned long long>(s.counters), features);
-    } else {
+    } else if (kWalkQueries == 1) {
         walk_f32_kernel<false><<<grid, kThreads, 0, st>>>(
             B<NodeF32>(s.n32), static_cast<int>(s.nodes), B<float4>(s.s4), B<uint32_t>(s.tperm),
             ni, static_cast<float>(g->gravitational_constant),
             static_cast<float>(g->softening_squared), static_cast<float>(theta2), e->ax, e->ay,
             e->az, B<unsigned long long>(s.counters), nullptr);
+    } else {
+        const unsigned mblocks = grid_for(s.n, kThreads * kWalkQueries);
+        const unsigned mgrid = (mblocks + kXcds - 1) / kXcds * kXcds;
+        walk_f32_multi_kernel<kWalkQueries><<<mgrid, kThreads, 0, st>>>(
+            B<NodeF32>(s.n32), static_cast<int>(s.nodes), B<float4>(s.s4), B<uint32_t>(s.tperm),
+            ni, static_cast<float>(g->gravitational_constant),
+            static_cast<float>(g->softening_squared), static_cast<float>(theta2), e->ax, e->ay,
+            e->az, B<unsigned long long>(s.counters));
     }
     NBX_HIP(error, hipGetLastError());
     NBX_HIP(error, hipEventRecord(e->ev[6], st));

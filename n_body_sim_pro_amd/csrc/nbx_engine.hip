@@ -643,8 +643,8 @@ NBodySimProStatus nbx_engine_bh_export_nodes(NBXEngine* engine, double* rows,
     NBX_HIP(error, hipMemcpy(start.data(), s.nstart.p, nodes * 4, hipMemcpyDeviceToHost));
     NBX_HIP(error, hipMemcpy(count.data(), s.ncount.p, nodes * 4, hipMemcpyDeviceToHost));
     {
-        std::vector<nbx::NodeF64> recs(nodes);
-        NBX_HIP(error, hipMemcpy(recs.data(), s.n64.p, nodes * sizeof(nbx::NodeF64),
+        std::vector<nbx::NodeF32> recs(nodes);
+        NBX_HIP(error, hipMemcpy(recs.data(), s.n32.p, nodes * sizeof(nbx::NodeF32),
                                  hipMemcpyDeviceToHost));
         for (size_t k = 0; k < nodes; ++k) leaf[k] = recs[k].leaf;
     }

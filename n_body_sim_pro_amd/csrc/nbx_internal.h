@@ -91,6 +91,7 @@ struct BHState {
     int max_depth = 0;
     bool tie_path = false;
     bool mismatch_path = false;
+    bool has_n64 = false;     // the last build wrote FP64 node records
     double root[4] = {0, 0, 0, 1};
     // build inputs (FP64 x, y, z, m): either the master arrays or FP32-rounded copies
     Buf b4;                   // (x, y, z, m) per particle in original order
@@ -156,9 +157,12 @@ NBodySimProStatus bh_learned(Engine* e, const NBodySimProGravity* g, NBodySimPro
 
 // Stable LSD radix sort of (64-bit key, 32-bit value) pairs, ascending key;
 // equal keys keep their input order. `bits` = number of low key bits to sort.
+// The result goes to keys_out/vals_out when given (they must not alias the
+// alternate buffers), else back into keys/vals.
 NBodySimProStatus radix_sort_pairs(hipStream_t s, uint64_t* keys, uint32_t* vals,
                                    uint64_t* keys_alt, uint32_t* vals_alt, size_t n, int bits,
-                                   Buf& hist, NBodySimProError* error);
+                                   Buf& hist, NBodySimProError* error,
+                                   uint64_t* keys_out = nullptr, uint32_t* vals_out = nullptr);
 // Exclusive scan of uint32 (total written to out[n]).
 NBodySimProStatus exclusive_scan_u32(hipStream_t s, const uint32_t* in, uint32_t* out, size_t n,
                                      Buf& partial, NBodySimProError* error);

@@ -11,7 +11,8 @@
 // samples (K = samples) and take their operands through a 16x16 LDS transpose.
 //
 // Rounding points (mirrored by the CPU oracle in oracle/mlp_oracle.py):
-//   xb = bf16((f - mean) / std); z1 = W1b xb + b1; h1b = bf16(silu(z1));
+//   xb = bf16((f - mean) * inv_std), inv_std = float32(1 / std);
+//   z1 = W1b xb + b1; h1b = bf16(silu(z1));
 //   z2 = W2b h1b + b2; h2b = bf16(silu(z2)); y = W3b h2b + b3;
 //   dy = 2 (y - t) / B; dyb = bf16(dy); gW3 = dyb h2b^T; dh2 = W3b^T dyb;
 //   dz2 = dh2 silu'(z2); dz2b = bf16(dz2); gW2 = dz2b h1b^T; dh1 = W2b^T dz2b;
@@ -45,14 +46,9 @@ constexpr int kPartial = kParams + 1;  // + loss
 constexpr int kMlpThreads = 256;
 constexpr int kWavesPerBlock = kMlpThreads / kWave;
 
+// FP32 -> BF16, round to nearest even (v_cvt_pk_bf16_f32 on gfx950).
 __device__ inline short f2bf(float x) {
-    // round to nearest even; NaN stays NaN
-    uint32_t u = __float_as_uint(x);
-    if ((u & 0x7FFFFFFFu) > 0x7F800000u) {
-        return static_cast<short>(0x7FC0);
-    }
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return static_cast<short>(u >> 16);
+    return __builtin_bit_cast(short, static_cast<__bf16>(x));
 }
 
 __device__ inline float bf2f(short h) {
@@ -64,7 +60,7 @@ __device__ inline f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
 }
 
 __device__ inline float sigmoidf_(float z) {
-    return 1.f / (1.f + __expf(-z));
+    return __builtin_amdgcn_rcpf(1.f + __expf(-z));
 }
 
 __device__ inline bf16x4 pack(float a, float b, float c, float d) {
@@ -125,13 +121,13 @@ __device__ inline void load_weights(const float* __restrict__ P, int lane, Weigh
 // 4*(lane/16) .. +3; also returns the FP32 values for the LDS transpose.
 __device__ inline bf16x4 load_x(const float* __restrict__ feats, const int* __restrict__ index,
                                 int sample, bool valid, const float* __restrict__ mean,
-                                const float* __restrict__ stdv, int lane, float xs[4]) {
+                                const float* __restrict__ inv_std, int lane, float xs[4]) {
     const int f0 = 4 * (lane >> 4);
     const int s = index != nullptr && valid ? index[sample] : sample;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         const float raw = valid ? feats[static_cast<size_t>(s) * kF + f0 + v] : 0.f;
-        xs[v] = valid ? (raw - mean[f0 + v]) / stdv[f0 + v] : 0.f;
+        xs[v] = valid ? (raw - mean[f0 + v]) * inv_std[f0 + v] : 0.f;
     }
     return pack(xs[0], xs[1], xs[2], xs[3]);
 }
@@ -170,7 +166,7 @@ __device__ inline void forward(const Weights& w, bf16x4 xb, Forward& f) {
 
 // Inference: raw features (n x 16) -> y (n x 3).
 __global__ __launch_bounds__(kMlpThreads) void mlp_forward_kernel(
-    const float* __restrict__ P, const float* __restrict__ mean, const float* __restrict__ stdv,
+    const float* __restrict__ P, const float* __restrict__ mean, const float* __restrict__ inv_std,
     const float* __restrict__ feats, int n, float* __restrict__ out) {
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = (blockIdx.x * kMlpThreads + threadIdx.x) / kWave;
@@ -182,7 +178,7 @@ __global__ __launch_bounds__(kMlpThreads) void mlp_forward_kernel(
         const int sample = 16 * g + (lane & 15);
         const bool valid = sample < n;
         float xs[4];
-        const bf16x4 xb = load_x(feats, nullptr, sample, valid, mean, stdv, lane, xs);
+        const bf16x4 xb = load_x(feats, nullptr, sample, valid, mean, inv_std, lane, xs);
         Forward f;
         forward(w, xb, f);
         if (valid && lane < 16) {
@@ -228,7 +224,7 @@ __device__ inline void bf_vals(bf16x4 b, float out[4]) {
 // Gradients over `n` samples (optionally through `index`), one FP32 partial
 // per wave: [gradients (kParams), loss].
 __global__ __launch_bounds__(kMlpThreads) void mlp_grad_kernel(
-    const float* __restrict__ P, const float* __restrict__ mean, const float* __restrict__ stdv,
+    const float* __restrict__ P, const float* __restrict__ mean, const float* __restrict__ inv_std,
     const float* __restrict__ feats, const float* __restrict__ targets,
     const int* __restrict__ index, int n, float inv_batch, float* __restrict__ partial) {
     __shared__ float lds_all[kWavesPerBlock][16 * 17];
@@ -270,7 +266,7 @@ __global__ __launch_bounds__(kMlpThreads) void mlp_grad_kernel(
         const int sample = 16 * g + (lane & 15);
         const bool valid = sample < n;
         float xs[4];
-        const bf16x4 xb = load_x(feats, index, sample, valid, mean, stdv, lane, xs);
+        const bf16x4 xb = load_x(feats, index, sample, valid, mean, inv_std, lane, xs);
         Forward f;
         forward(w, xb, f);
         // dL/dy for outputs o = 4*(lane/16)+v < 3 of sample lane % 16
@@ -479,7 +475,7 @@ NBodySimProStatus mlp_gradients_device(Mlp& m, const float* feats, const float* 
     const int waves = grad_waves(m, n);
     NBX_TRY(m.partial.need<float>(static_cast<size_t>(waves) * kPartial, error));
     mlp_grad_kernel<<<waves / kWavesPerBlock, kMlpThreads, 0, m.stream>>>(
-        m.params, m.norm, m.norm + kF, feats, targets, index, static_cast<int>(n),
+        m.params, m.norm, m.norm + 2 * kF, feats, targets, index, static_cast<int>(n),
         1.f / static_cast<float>(n), m.partial.as<float>());
     mlp_reduce_kernel<<<grid_for(kPartial, 256), 256, 0, m.stream>>>(m.partial.as<float>(), waves,
                                                                      m.grads);
@@ -493,7 +489,7 @@ NBodySimProStatus mlp_forward_device(Mlp& m, const float* feats, size_t n, float
     int blocks = (groups + kWavesPerBlock - 1) / kWavesPerBlock;
     blocks = blocks < m.cu_count * 8 ? blocks : m.cu_count * 8;
     mlp_forward_kernel<<<blocks > 0 ? blocks : 1, kMlpThreads, 0, m.stream>>>(
-        m.params, m.norm, m.norm + kF, feats, static_cast<int>(n), out);
+        m.params, m.norm, m.norm + 2 * kF, feats, static_cast<int>(n), out);
     NBX_HIP(error, hipGetLastError());
     return N_BODY_SIM_PRO_STATUS_OK;
 }
@@ -527,9 +523,10 @@ __device__ inline void local_frame(const float* __restrict__ wf, Frame& fr) {
     const float a = sqrtf(ax * ax + ay * ay + az * az);
     fr.a = a;
     if (a > 0.f) {
-        fr.e1[0] = ax / a;
-        fr.e1[1] = ay / a;
-        fr.e1[2] = az / a;
+        const float ia = __builtin_amdgcn_rcpf(a);
+        fr.e1[0] = ax * ia;
+        fr.e1[1] = ay * ia;
+        fr.e1[2] = az * ia;
     } else {
         fr.e1[0] = 1.f;
         fr.e1[1] = 0.f;
@@ -558,10 +555,10 @@ __device__ inline void local_frame(const float* __restrict__ wf, Frame& fr) {
     px -= q * fr.e1[0];
     py -= q * fr.e1[1];
     pz -= q * fr.e1[2];
-    pn = sqrtf(px * px + py * py + pz * pz);
-    fr.e2[0] = px / pn;
-    fr.e2[1] = py / pn;
-    fr.e2[2] = pz / pn;
+    const float ipn = __builtin_amdgcn_rcpf(sqrtf(px * px + py * py + pz * pz));
+    fr.e2[0] = px * ipn;
+    fr.e2[1] = py * ipn;
+    fr.e2[2] = pz * ipn;
     fr.e3[0] = fr.e1[1] * fr.e2[2] - fr.e1[2] * fr.e2[1];
     fr.e3[1] = fr.e1[2] * fr.e2[0] - fr.e1[0] * fr.e2[2];
     fr.e3[2] = fr.e1[0] * fr.e2[1] - fr.e1[1] * fr.e2[0];
@@ -585,25 +582,25 @@ __device__ inline void sym3_eig_min_max(float xx, float yy, float zz, float xy, 
         lmin = lmax = q;
         return;
     }
-    const float ip = 1.f / p;
+    const float ip = __builtin_amdgcn_rcpf(p);
     const float bxx = ax * ip, byy = ay * ip, bzz = az * ip;
     const float bxy = xy * ip, bxz = xz * ip, byz = yz * ip;
     const float det = bxx * (byy * bzz - byz * byz) - bxy * (bxy * bzz - byz * bxz) +
                       bxz * (bxy * byz - byy * bxz);
     const float r = fminf(1.f, fmaxf(-1.f, 0.5f * det));
     const float phi = acosf(r) / 3.f;
-    lmax = q + 2.f * p * cosf(phi);
-    lmin = q + 2.f * p * cosf(phi + 2.0943951f);
+    lmax = q + 2.f * p * __cosf(phi);
+    lmin = q + 2.f * p * __cosf(phi + 2.0943951f);
 }
 
 // Raw features (before standardisation) from one walk record.
 __device__ inline void raw_features(const float* __restrict__ wf, const Frame& fr, float m,
                                     int leaf_depth, float log2_root_edge, float f[kF]) {
     const float cells = wf[3], leaves = wf[7];
-    const float inv = fr.a > 0.f ? 1.f / fr.a : 0.f;
+    const float inv = fr.a > 0.f ? __builtin_amdgcn_rcpf(fr.a) : 0.f;
     const float wxx = wf[8], wyy = wf[9], wzz = wf[10], wxy = wf[11], wxz = wf[12], wyz = wf[13];
     const float tr = wxx + wyy + wzz;
-    const float itr = tr > 0.f ? 1.f / tr : 0.f;
+    const float itr = tr > 0.f ? __builtin_amdgcn_rcpf(tr) : 0.f;
     // e^T W e for the frame axes
     const float a0 = fr.e1[0], a1 = fr.e1[1], a2 = fr.e1[2];
     const float b0 = fr.e2[0], b1 = fr.e2[1], b2 = fr.e2[2];
@@ -613,7 +610,7 @@ __device__ inline void raw_features(const float* __restrict__ wf, const Frame& f
                      2.f * (wxy * b0 * b1 + wxz * b0 * b2 + wyz * b1 * b2);
     float lmin = 0.f, lmax = 0.f;
     sym3_eig_min_max(wxx, wyy, wzz, wxy, wxz, wyz, lmin, lmax);
-    f[0] = cells > 0.f ? sqrtf(wf[14] / cells) : 0.f;   // rms distance / cell edge
+    f[0] = cells > 0.f ? sqrtf(wf[14] * __builtin_amdgcn_rcpf(cells)) : 0.f;   // rms distance / edge
     f[1] = cells > 0.f ? sqrtf(wf[15]) : 0.f;           // minimum distance / cell edge
     f[2] = log2f(tr * inv + 0x1p-40f);                  // truncation scale / |a|
     // own mass over the leaf cell volume: edge = 2 h_root 2^-depth
@@ -675,7 +672,7 @@ __global__ void features_kernel(const float* __restrict__ walkf, const float4* _
 // assembles one query's features into LDS, four 16-query MFMA passes run the
 // network, and every lane applies its own query's correction.
 __global__ __launch_bounds__(kMlpThreads) void learned_apply_kernel(
-    const float* __restrict__ P, const float* __restrict__ mean, const float* __restrict__ stdv,
+    const float* __restrict__ P, const float* __restrict__ mean, const float* __restrict__ inv_std,
     float out_scale, const float* __restrict__ walkf, const float4* __restrict__ s4,
     const int32_t* __restrict__ ndepth, const uint32_t* __restrict__ scan,
     const uint32_t* __restrict__ cnt, const uint32_t* __restrict__ tperm, int n,
@@ -713,7 +710,7 @@ __global__ __launch_bounds__(kMlpThreads) void learned_apply_kernel(
         }
 #pragma unroll
         for (int k = 0; k < kF; ++k) {
-            xs[lane][k] = (f[k] - mean[k]) / stdv[k];
+            xs[lane][k] = (f[k] - mean[k]) * inv_std[k];
         }
         wave_lds_sync();
         const int f0 = 4 * (lane >> 4);
@@ -757,7 +754,7 @@ NBodySimProStatus bh_learned(Engine* e, const NBodySimProGravity* g, NBodySimPro
     int blocks = (tiles + kWavesPerBlock - 1) / kWavesPerBlock;
     blocks = blocks < m->cu_count * 4 ? blocks : m->cu_count * 4;
     learned_apply_kernel<<<blocks, kMlpThreads, 0, e->stream>>>(
-        m->params, m->norm, m->norm + kF, m->out_scale, walkf, s.s4.as<float4>(),
+        m->params, m->norm, m->norm + 2 * kF, m->out_scale, walkf, s.s4.as<float4>(),
         s.ndepth.as<int32_t>(), s.scan.as<uint32_t>(), s.cnt.as<uint32_t>(),
         s.tperm.as<uint32_t>(), static_cast<int>(s.n),
         static_cast<float>(std::log2(2.0 * s.root[3])),
@@ -845,7 +842,7 @@ NBXMlp* nbx_mlp_create(int device, NBodySimProError* error) {
               hipStreamCreateWithFlags(&m.stream, hipStreamNonBlocking) == hipSuccess;
     if (ok) m.cu_count = prop.multiProcessorCount;
     ok = ok && hipMalloc(reinterpret_cast<void**>(&m.params), nbx::kParams * sizeof(float)) == hipSuccess;
-    ok = ok && hipMalloc(reinterpret_cast<void**>(&m.norm), 2 * nbx::kF * sizeof(float)) == hipSuccess;
+    ok = ok && hipMalloc(reinterpret_cast<void**>(&m.norm), 3 * nbx::kF * sizeof(float)) == hipSuccess;
     ok = ok && hipMalloc(reinterpret_cast<void**>(&m.grads), nbx::kPartial * sizeof(float)) == hipSuccess;
     ok = ok && hipMalloc(reinterpret_cast<void**>(&m.adam_m), nbx::kParams * sizeof(float)) == hipSuccess;
     ok = ok && hipMalloc(reinterpret_cast<void**>(&m.adam_v), nbx::kParams * sizeof(float)) == hipSuccess;
@@ -853,8 +850,9 @@ NBXMlp* nbx_mlp_create(int device, NBodySimProError* error) {
          hipMemset(m.adam_m, 0, nbx::kParams * sizeof(float)) == hipSuccess &&
          hipMemset(m.adam_v, 0, nbx::kParams * sizeof(float)) == hipSuccess;
     if (ok) {
-        std::vector<float> norm(2 * nbx::kF, 0.f);
-        for (int k = 0; k < nbx::kF; ++k) norm[nbx::kF + k] = 1.f;
+        // mean, std, 1 / std
+        std::vector<float> norm(3 * nbx::kF, 1.f);
+        for (int k = 0; k < nbx::kF; ++k) norm[k] = 0.f;
         ok = hipMemcpy(m.norm, norm.data(), norm.size() * sizeof(float), hipMemcpyHostToDevice) ==
              hipSuccess;
     }
@@ -883,6 +881,11 @@ NBodySimProStatus nbx_mlp_set_params(NBXMlp* p, const float* params, const float
     NBX_HIP(error, hipMemcpy(m.params, params, nbx::kParams * sizeof(float), hipMemcpyHostToDevice));
     NBX_HIP(error, hipMemcpy(m.norm, mean, nbx::kF * sizeof(float), hipMemcpyHostToDevice));
     NBX_HIP(error, hipMemcpy(m.norm + nbx::kF, stdv, nbx::kF * sizeof(float), hipMemcpyHostToDevice));
+    float inv[nbx::kF];
+    for (int k = 0; k < nbx::kF; ++k) {
+        inv[k] = 1.f / stdv[k];   // +inf -> 0: the feature is ignored
+    }
+    NBX_HIP(error, hipMemcpy(m.norm + 2 * nbx::kF, inv, sizeof(inv), hipMemcpyHostToDevice));
     NBX_HIP(error, hipMemset(m.adam_m, 0, nbx::kParams * sizeof(float)));
     NBX_HIP(error, hipMemset(m.adam_v, 0, nbx::kParams * sizeof(float)));
     m.adam_step = 0;

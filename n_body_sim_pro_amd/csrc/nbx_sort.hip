@@ -4,19 +4,31 @@
 // The reference sorts a size_t permutation by 63-bit Morton keys with three
 // stable 21-bit LSD passes and a 2^21-entry counting array
 // (src/c/barnes_hut/barnes_hut.c:312-339). Any stable sort on the full key
-// yields the same permutation (ties keep ascending original index), so here
-// the digit is 8 bits: 256 buckets fit one LDS histogram per tile and the
-// per-tile offsets stay small, which suits a 256-CU device better than a
-// 16 MiB counting array.
+// yields the same permutation (ties keep ascending original index). Here the
+// digit is 9 bits (7 passes cover 63 bits) and a pass is three launches:
+//   sort_histogram   per-tile digit counts (4096-item tiles, LDS atomics);
+//   sort_scan_digit  per digit, exclusive scan over tiles (+ digit totals);
+//   sort_scatter     each tile ranks its items stably in LDS (wave ballots
+//                    match equal digits; waves and rounds keep input order),
+//                    stages them digit-sorted in LDS, then writes each
+//                    digit's run to its global slot with consecutive lanes
+//                    on consecutive addresses.
 #include "nbx_internal.h"
 
 namespace nbx {
 namespace {
 
 constexpr int kSortThreads = 256;
-constexpr int kSortIpt = 8;
-constexpr int kSortTile = kSortThreads * kSortIpt;
-constexpr int kRadix = 256;
+constexpr int kSortWaves = kSortThreads / kWave;
+constexpr int kSortIpt = 16;
+constexpr int kSortTile = kSortThreads * kSortIpt;   // 4096
+constexpr int kDigitBits = 9;
+constexpr int kRadix = 1 << kDigitBits;             // 512
+constexpr int kDigitsPerThread = kRadix / kSortThreads;
+
+__device__ inline unsigned digit_of(uint64_t key, int shift) {
+    return static_cast<unsigned>((key >> shift) & (kRadix - 1));
+}
 
 __global__ __launch_bounds__(kSortThreads) void sort_histogram(const uint64_t* __restrict__ keys,
                                                               size_t n, int shift,
@@ -24,42 +36,72 @@ __global__ __launch_bounds__(kSortThreads) void sort_histogram(const uint64_t* _
                                                               unsigned tiles) {
     __shared__ uint32_t h[kRadix];
     const unsigned tile = blockIdx.x;
-    h[threadIdx.x] = 0;
+    for (int k = threadIdx.x; k < kRadix; k += kSortThreads) {
+        h[k] = 0;
+    }
     __syncthreads();
     const size_t base = static_cast<size_t>(tile) * kSortTile;
-#pragma unroll
+#pragma unroll 4
     for (int r = 0; r < kSortIpt; ++r) {
         const size_t i = base + static_cast<size_t>(r) * kSortThreads + threadIdx.x;
         if (i < n) {
-            atomicAdd(&h[(keys[i] >> shift) & 0xFFu], 1u);
+            atomicAdd(&h[digit_of(keys[i], shift)], 1u);
         }
     }
     __syncthreads();
-    hist[static_cast<size_t>(threadIdx.x) * tiles + tile] = h[threadIdx.x];
+    for (int k = threadIdx.x; k < kRadix; k += kSortThreads) {
+        hist[static_cast<size_t>(k) * tiles + tile] = h[k];
+    }
+}
+
+// Inclusive scan over one wave.
+__device__ inline uint32_t wave_inclusive_scan(uint32_t v, unsigned lane) {
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const uint32_t t = __shfl_up(v, off);
+        if (lane >= static_cast<unsigned>(off)) {
+            v += t;
+        }
+    }
+    return v;
+}
+
+// Exclusive scan over the block's 256 threads; `total` gets the block sum.
+// `wsum` holds kSortWaves entries; the caller syncs before reusing it.
+__device__ inline uint32_t block_exclusive_scan256(uint32_t v, uint32_t* wsum, uint32_t& total) {
+    const unsigned lane = threadIdx.x & (kWave - 1);
+    const unsigned wave = threadIdx.x / kWave;
+    const uint32_t incl = wave_inclusive_scan(v, lane);
+    if (lane == kWave - 1) {
+        wsum[wave] = incl;
+    }
+    __syncthreads();
+    uint32_t before = 0;
+    total = 0;
+#pragma unroll
+    for (int w = 0; w < kSortWaves; ++w) {
+        before += static_cast<unsigned>(w) < wave ? wsum[w] : 0u;
+        total += wsum[w];
+    }
+    return before + incl - v;
 }
 
 // One block per digit: exclusive scan of that digit's per-tile counts.
 __global__ __launch_bounds__(kSortThreads) void sort_scan_digit(uint32_t* __restrict__ hist,
                                                                unsigned tiles,
                                                                uint32_t* __restrict__ totals) {
-    __shared__ uint32_t s[kSortThreads];
+    __shared__ uint32_t wsum[kSortWaves];
     uint32_t* row = hist + static_cast<size_t>(blockIdx.x) * tiles;
     uint32_t carry = 0;
     for (unsigned start = 0; start < tiles; start += kSortThreads) {
         const unsigned i = start + threadIdx.x;
         const uint32_t v = i < tiles ? row[i] : 0u;
-        s[threadIdx.x] = v;
-        __syncthreads();
-        for (int off = 1; off < kSortThreads; off <<= 1) {
-            const uint32_t t = threadIdx.x >= static_cast<unsigned>(off) ? s[threadIdx.x - off] : 0u;
-            __syncthreads();
-            s[threadIdx.x] += t;
-            __syncthreads();
-        }
+        uint32_t total;
+        const uint32_t ex = block_exclusive_scan256(v, wsum, total);
         if (i < tiles) {
-            row[i] = carry + s[threadIdx.x] - v;
+            row[i] = carry + ex;
         }
-        carry += s[kSortThreads - 1];
+        carry += total;
         __syncthreads();
     }
     if (threadIdx.x == 0) {
@@ -71,29 +113,53 @@ __global__ __launch_bounds__(kSortThreads) void sort_scatter(
     const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
     uint64_t* __restrict__ kout, uint32_t* __restrict__ vout, size_t n, int shift,
     const uint32_t* __restrict__ hist, unsigned tiles, const uint32_t* __restrict__ totals) {
-    __shared__ uint32_t base[kRadix];
-    __shared__ uint32_t running[kRadix];
-    __shared__ uint32_t wcount[kSortThreads / kWave][kRadix];
+    __shared__ uint64_t skey[kSortTile];
+    __shared__ uint32_t sval[kSortTile];
+    __shared__ uint32_t gbase[kRadix];     // global slot of this tile's first item of digit d
+    __shared__ uint32_t lbase[kRadix];     // start of digit d in the tile's staged order
+    __shared__ uint32_t running[kRadix];   // items of digit d staged so far
+    __shared__ uint32_t wcount[kSortWaves][kRadix];
+    __shared__ uint32_t wsum[kSortWaves];
     const unsigned tid = threadIdx.x;
     const unsigned lane = tid & (kWave - 1);
     const unsigned wave = tid / kWave;
     const unsigned tile = blockIdx.x;
 
-    // Digit bases: exclusive scan of the digit totals.
-    base[tid] = totals[tid];
-    __syncthreads();
-    for (int off = 1; off < kRadix; off <<= 1) {
-        const uint32_t t = tid >= static_cast<unsigned>(off) ? base[tid - off] : 0u;
-        __syncthreads();
-        base[tid] += t;
-        __syncthreads();
+    // Digit starts over the whole array and within this tile; each thread
+    // owns kDigitsPerThread consecutive digits.
+    uint32_t tot[kDigitsPerThread], cnt[kDigitsPerThread], off[kDigitsPerThread];
+    uint32_t tsum = 0, csum = 0;
+#pragma unroll
+    for (int j = 0; j < kDigitsPerThread; ++j) {
+        const unsigned d = tid * kDigitsPerThread + j;
+        tot[j] = totals[d];
+        off[j] = hist[static_cast<size_t>(d) * tiles + tile];
+        const uint32_t next = tile + 1 < tiles ? hist[static_cast<size_t>(d) * tiles + tile + 1]
+                                               : tot[j];
+        cnt[j] = next - off[j];
+        tsum += tot[j];
+        csum += cnt[j];
     }
-    const uint32_t my_total = totals[tid];
+    uint32_t total;
+    uint32_t gstart = block_exclusive_scan256(tsum, wsum, total);
     __syncthreads();
-    base[tid] = base[tid] - my_total + hist[static_cast<size_t>(tid) * tiles + tile];
-    running[tid] = 0;
+    uint32_t lstart = block_exclusive_scan256(csum, wsum, total);
+#pragma unroll
+    for (int j = 0; j < kDigitsPerThread; ++j) {
+        const unsigned d = tid * kDigitsPerThread + j;
+        gbase[d] = gstart + off[j];
+        lbase[d] = lstart;
+        running[d] = 0;
+        gstart += tot[j];
+        lstart += cnt[j];
+    }
+    for (int k = tid; k < kSortWaves * kRadix; k += kSortThreads) {
+        (&wcount[0][0])[k] = 0;
+    }
     __syncthreads();
 
+    // Stable ranks: rounds in input order, waves in order within a round,
+    // lanes in order within a wave.
     const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (kWave - lane));
     const size_t tbase = static_cast<size_t>(tile) * kSortTile;
     for (int r = 0; r < kSortIpt; ++r) {
@@ -105,41 +171,49 @@ __global__ __launch_bounds__(kSortThreads) void sort_scatter(
         if (valid) {
             key = kin[i];
             val = vin[i];
-            d = static_cast<unsigned>((key >> shift) & 0xFFu);
+            d = digit_of(key, shift);
         }
-        // Lanes of this wave holding the same digit.
         uint64_t peers = __ballot(valid);
 #pragma unroll
-        for (int b = 0; b < 8; ++b) {
+        for (int b = 0; b < kDigitBits; ++b) {
             const bool bit = (d >> b) & 1u;
             const uint64_t bal = __ballot(bit);
             peers &= bit ? bal : ~bal;
         }
-        const unsigned rank_in_wave = __popcll(peers & lt);
-        for (unsigned k = tid; k < (kSortThreads / kWave) * kRadix; k += kSortThreads) {
-            (&wcount[0][0])[k] = 0;
-        }
-        __syncthreads();
-        if (valid && rank_in_wave == 0) {
-            wcount[wave][d] = static_cast<uint32_t>(__popcll(peers));
+        const unsigned rank = __popcll(peers & lt);
+        const bool leader = valid && rank == 0;
+        const uint32_t group = static_cast<uint32_t>(__popcll(peers));
+        if (leader) {
+            wcount[wave][d] = group;
         }
         __syncthreads();
         if (valid) {
-            uint32_t off = running[d] + rank_in_wave;
+            uint32_t o = running[d] + rank;
             for (unsigned w = 0; w < wave; ++w) {
-                off += wcount[w][d];
+                o += wcount[w][d];
             }
-            const uint32_t dst = base[d] + off;
-            kout[dst] = key;
-            vout[dst] = val;
+            const uint32_t pos = lbase[d] + o;
+            skey[pos] = key;
+            sval[pos] = val;
         }
         __syncthreads();
-        uint32_t add = 0;
-        for (unsigned w = 0; w < kSortThreads / kWave; ++w) {
-            add += wcount[w][tid];
+        if (leader) {
+            atomicAdd(&running[d], group);
+            wcount[wave][d] = 0;
         }
-        running[tid] += add;
         __syncthreads();
+    }
+
+    // Staged order is digit-sorted: write each digit's run contiguously.
+    const unsigned m = static_cast<unsigned>(n - tbase < static_cast<size_t>(kSortTile)
+                                                 ? n - tbase
+                                                 : static_cast<size_t>(kSortTile));
+    for (unsigned idx = tid; idx < m; idx += kSortThreads) {
+        const uint64_t key = skey[idx];
+        const unsigned d = digit_of(key, shift);
+        const uint32_t dst = gbase[d] + (idx - lbase[d]);
+        kout[dst] = key;
+        vout[dst] = sval[idx];
     }
 }
 
@@ -230,39 +304,53 @@ __global__ __launch_bounds__(kScanThreads) void scan_apply(const uint32_t* __res
 
 NBodySimProStatus radix_sort_pairs(hipStream_t s, uint64_t* keys, uint32_t* vals,
                                    uint64_t* keys_alt, uint32_t* vals_alt, size_t n, int bits,
-                                   Buf& hist, NBodySimProError* error) {
-    if (n <= 1) {
+                                   Buf& hist, NBodySimProError* error, uint64_t* keys_out,
+                                   uint32_t* vals_out) {
+    if (keys_out == nullptr || vals_out == nullptr) {
+        keys_out = keys;
+        vals_out = vals;
+    }
+    if (n == 0) {
         return N_BODY_SIM_PRO_STATUS_OK;
     }
     if (n > 0xFFFFFFFFull) {
         NBX_FAIL(error, N_BODY_SIM_PRO_STATUS_INVALID_ARGUMENT, "sort size exceeds 2^32");
     }
+    if (n == 1) {
+        if (keys_out != keys) {
+            NBX_HIP(error, hipMemcpyAsync(keys_out, keys, sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+            NBX_HIP(error, hipMemcpyAsync(vals_out, vals, sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+        }
+        return N_BODY_SIM_PRO_STATUS_OK;
+    }
     const unsigned tiles = grid_for(n, kSortTile);
     NBX_TRY(hist.need<uint32_t>(static_cast<size_t>(kRadix) * tiles + kRadix, error));
     uint32_t* h = hist.as<uint32_t>();
     uint32_t* totals = h + static_cast<size_t>(kRadix) * tiles;
-    const int passes = (bits + 7) / 8;
-    uint64_t* kin = keys;
-    uint32_t* vin = vals;
-    uint64_t* kout = keys_alt;
-    uint32_t* vout = vals_alt;
+    const int passes = (bits + kDigitBits - 1) / kDigitBits;
+    const uint64_t* kin = keys;
+    const uint32_t* vin = vals;
     for (int p = 0; p < passes; ++p) {
-        const int shift = 8 * p;
+        const int shift = kDigitBits * p;
+        // ping-pong between the two buffers; the last pass writes the output
+        const bool last = p == passes - 1;
+        uint64_t* kout = last ? keys_out : ((p & 1) ? keys : keys_alt);
+        uint32_t* vout = last ? vals_out : ((p & 1) ? vals : vals_alt);
+        if (last && (kout == kin || static_cast<const uint32_t*>(vout) == vin)) {
+            kout = (kin == keys) ? keys_alt : keys;     // output aliases the input: one more hop
+            vout = (vin == vals) ? vals_alt : vals;
+        }
         sort_histogram<<<tiles, kSortThreads, 0, s>>>(kin, n, shift, h, tiles);
         sort_scan_digit<<<kRadix, kSortThreads, 0, s>>>(h, tiles, totals);
         sort_scatter<<<tiles, kSortThreads, 0, s>>>(kin, vin, kout, vout, n, shift, h, tiles,
                                                     totals);
-        uint64_t* tk = kin;
         kin = kout;
-        kout = tk;
-        uint32_t* tv = vin;
         vin = vout;
-        vout = tv;
     }
     NBX_HIP(error, hipGetLastError());
-    if (kin != keys) {
-        NBX_HIP(error, hipMemcpyAsync(keys, kin, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
-        NBX_HIP(error, hipMemcpyAsync(vals, vin, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+    if (kin != keys_out) {
+        NBX_HIP(error, hipMemcpyAsync(keys_out, kin, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+        NBX_HIP(error, hipMemcpyAsync(vals_out, vin, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
     }
     return N_BODY_SIM_PRO_STATUS_OK;
 }
