@@ -44,7 +44,9 @@ typedef enum NBXAlgorithm {
      * positions/masses, tile-compensated FP32 accumulation. */
     NBX_ALGORITHM_DIRECT_FP32 = 1,
     /* hip_barnes_hut: octree built from FP32-rounded positions and masses
-     * (keys, order and topology exact in FP64 on those values), FP32 walk. */
+     * (keys, order and topology exact in FP64 on those values), FP32 walk:
+     * the group walk by default, per-particle reference lists with
+     * NBX_FLAG_BH_REFERENCE_LISTS. */
     NBX_ALGORITHM_BARNES_HUT = 2,
     /* hip_barnes_hut_fp64: same device tree on the FP64 positions with an
      * FP64 walk in the reference's visit order: bit-identical to
@@ -55,11 +57,22 @@ typedef enum NBXAlgorithm {
     NBX_ALGORITHM_BH_LEARNED = 4
 } NBXAlgorithm;
 
+/* NBXConfig.flags bits. */
+enum {
+    /* hip_barnes_hut walks every particle with its own reference interaction
+     * list (the reference's acceptance per particle, reference counts in the
+     * stats) instead of the default group walk, in which each wave of 64
+     * consecutive particles shares one list accepted against the group's
+     * bounding box (every particle's list then refines its reference list:
+     * more interactions, same or smaller error). */
+    NBX_FLAG_BH_REFERENCE_LISTS = 1u
+};
+
 typedef struct NBXConfig {
     size_t capacity;  /* maximum particle count */
     int device;       /* HIP device ordinal */
     double theta;     /* Barnes-Hut opening angle (reference default 0.7) */
-    uint32_t flags;   /* reserved, 0 */
+    uint32_t flags;   /* NBX_FLAG_* bits, 0 for the defaults */
 } NBXConfig;
 
 /* Per-evaluation timings from HIP events on the engine's stream (ms). */

@@ -27,7 +27,7 @@
  * Parameter layout (float, nbx_mlp_param_count() values):
  *   W1[64][16], b1[64], W2[64][64], b2[64], W3[16][64] (rows 3..15 unused,
  *   kept zero), b3[16] (entries 3..15 unused).
- * The model standardises features as (f - mean) / std and its output y is
+ * The model standardises features as (f - mean) * float32(1 / std); its output y is
  * scaled: residual / |a_BH| = (y0 e1 + y1 e2 + y2 e3) / out_scale.
  */
 #ifndef NBX_LEARNED_H

@@ -488,9 +488,11 @@ __device__ inline int cell_id(const uint64_t* w, const uint64_t* ext, const uint
     return internal_id(scan, delta, s, d);
 }
 
-// Static node fields, leaf records, the per-depth lists of internal nodes
-// (by internal id), and every node's slot in its parent's child table
-// kids[parent_internal_id * 8 + octant].
+// Static fields of the internal nodes, the walk records of the leaves, the
+// per-depth lists of internal nodes (by internal id), and every node's slot
+// in its parent's child table kids[parent_internal_id * 8 + octant]
+// (internal child: its node index; leaf child: -(tree position + 2); the
+// per-node SoA arrays hold internal nodes only).
 __global__ __launch_bounds__(kThreads) void fill_kernel(
     const uint64_t* __restrict__ w, const uint64_t* __restrict__ ext,
     const int32_t* __restrict__ delta, const uint32_t* __restrict__ cnt,
@@ -541,20 +543,13 @@ __global__ __launch_bounds__(kThreads) void fill_kernel(
     const int depth = 1 + (dprev > hi ? dprev : hi);
     const int li = leaf_index(scan, cnt, t);
     if (depth > 0) {
+        // leaves are referenced by tree position, tagged -(t + 2): the
+        // centre-of-mass pass reads them straight from the tree-order arrays
         const int parent = depth - 1 == hi && hi >= lo ? internal_id(scan, delta, t, hi)
                                                        : cell_id(w, ext, scan, delta, t, depth - 1);
-        kids[parent * 8 + digit_at(w, ext, t, depth - 1)] = li;
+        kids[parent * 8 + digit_at(w, ext, t, depth - 1)] = -(t + 2);
     }
     const double x = sx[t], y = sy[t], z = sz[t], m = sm[t];
-    nd.skip[li] = li + 1;
-    nd.depth[li] = depth;
-    nd.start[li] = t;
-    nd.count[li] = 1;
-    nd.cx[li] = x;
-    nd.cy[li] = y;
-    nd.cz[li] = z;
-    nd.m[li] = m;
-    nd.minpos[li] = t;
     if (n64 != nullptr) {   // FP64 records only for the exact FP64 walk
         NodeF64 a;
         a.cx = x;
@@ -587,9 +582,9 @@ __device__ inline void com_node(int iid, const int* __restrict__ inode,
                                 const int* __restrict__ kids, NodeSoA nd,
                                 const uint32_t* __restrict__ trank,
                                 const double* __restrict__ sx, const double* __restrict__ sy,
-                                const double* __restrict__ sz, const double4* __restrict__ b4,
-                                double root_h, NodeF64* __restrict__ n64,
-                                NodeF32* __restrict__ n32) {
+                                const double* __restrict__ sz, const double* __restrict__ sm,
+                                const double4* __restrict__ b4, double root_h,
+                                NodeF64* __restrict__ n64, NodeF32* __restrict__ n32) {
     const int p = inode[iid];
     const int end = nd.skip[p];
     int kid[8];
@@ -603,15 +598,29 @@ __device__ inline void com_node(int iid, const int* __restrict__ inode,
 #pragma unroll
     for (int o = 0; o < 8; ++o) {
         const int c = kid[o];
-        if (c < 0) {
+        if (c == -1) {
             continue;
         }
-        const double cm = nd.m[c];
+        double cm, cx, cy, cz;
+        int mp;
+        if (c < -1) {   // leaf at tree position -c - 2
+            const int t = -c - 2;
+            cm = sm[t];
+            cx = sx[t];
+            cy = sy[t];
+            cz = sz[t];
+            mp = t;
+        } else {
+            cm = nd.m[c];
+            cx = nd.cx[c];
+            cy = nd.cy[c];
+            cz = nd.cz[c];
+            mp = nd.minpos[c];
+        }
         mass += cm;
-        x += cm * nd.cx[c];
-        y += cm * nd.cy[c];
-        z += cm * nd.cz[c];
-        const int mp = nd.minpos[c];
+        x += cm * cx;
+        y += cm * cy;
+        z += cm * cz;
         const uint32_t r = trank[mp];
         if (r < minrank) {
             minrank = r;
@@ -668,11 +677,11 @@ __global__ __launch_bounds__(kThreads) void com_level_kernel(
     const int* __restrict__ list, int count, const int* __restrict__ inode,
     const int* __restrict__ kids, NodeSoA nd, const uint32_t* __restrict__ trank,
     const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
-    const double4* __restrict__ b4, double root_h, NodeF64* __restrict__ n64,
-    NodeF32* __restrict__ n32) {
+    const double* __restrict__ sm, const double4* __restrict__ b4, double root_h,
+    NodeF64* __restrict__ n64, NodeF32* __restrict__ n32) {
     const int k = blockIdx.x * kThreads + threadIdx.x;
     if (k < count) {
-        com_node(list[k], inode, kids, nd, trank, sx, sy, sz, b4, root_h, n64, n32);
+        com_node(list[k], inode, kids, nd, trank, sx, sy, sz, sm, b4, root_h, n64, n32);
     }
 }
 
@@ -683,13 +692,14 @@ __global__ __launch_bounds__(1024) void com_small_levels_kernel(
     const unsigned* __restrict__ dcount, int d_hi, int d_lo, const int* __restrict__ inode,
     const int* __restrict__ kids, NodeSoA nd,
     const uint32_t* __restrict__ trank, const double* __restrict__ sx,
-    const double* __restrict__ sy, const double* __restrict__ sz, const double4* __restrict__ b4,
-    double root_h, NodeF64* __restrict__ n64, NodeF32* __restrict__ n32) {
+    const double* __restrict__ sy, const double* __restrict__ sz, const double* __restrict__ sm,
+    const double4* __restrict__ b4, double root_h, NodeF64* __restrict__ n64,
+    NodeF32* __restrict__ n32) {
     for (int d = d_hi; d >= d_lo; --d) {
         const int count = static_cast<int>(dcount[d]);
         const int* lst = list + doff[d];
         for (int k = threadIdx.x; k < count; k += blockDim.x) {
-            com_node(lst[k], inode, kids, nd, trank, sx, sy, sz, b4, root_h, n64, n32);
+            com_node(lst[k], inode, kids, nd, trank, sx, sy, sz, sm, b4, root_h, n64, n32);
         }
         __syncthreads();
     }
@@ -780,141 +790,136 @@ constexpr int kXcds = 8;
 #ifndef NBX_WALK_QUERIES
 #define NBX_WALK_QUERIES 2
 #endif
-// Queries per lane of the FP32 walk (1: walk_f32_kernel, else walk_f32_multi_kernel).
+// Queries per lane of the FP32 walk.
 constexpr int kWalkQueries = NBX_WALK_QUERIES;
 __device__ inline int xcd_block(int b, int grid) {
     return (b % kXcds) * (grid / kXcds) + b / kXcds;
 }
 
-// FP32 variant: one node per wave-step, node fields in SGPRs (scalar loads of
-// a wave-uniform index). Lanes with nothing to do at a node are masked off,
-// and when no lane is active the arithmetic is skipped outright. G is
-// applied once at the end. FEATURES additionally records per-query walk
-// statistics for the learned correction. (Measured alternatives, slower on
-// MI355X: branch-free steps, vector-loaded nodes, prefetching both
-// successors; the walk is bound by node fetch latency, not VALU issue.)
+// Per-query accumulation of the FP32 walks: the softened interaction terms
+// and, with FEATURES, the walk statistics of the learned correction
+// (accepted-cell force, error-weighted direction tensor W, sum and minimum of
+// (distance / cell edge)^2). G is applied once, when the result is stored.
 template <bool FEATURES>
-__global__ __launch_bounds__(kThreads) void walk_f32_kernel(
-    const NodeF32* __restrict__ n32, int nodes, const float4* __restrict__ s4,
-    const uint32_t* __restrict__ tperm, int n, float G, float eps2, float theta2,
-    double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az,
-    unsigned long long* __restrict__ counters, float* __restrict__ features) {
-    const int t = xcd_block(blockIdx.x, gridDim.x) * kThreads + threadIdx.x;
-    const bool valid = t < n;
-    const float4 q = s4[valid ? t : n - 1];
-    int skip_until = valid ? 0 : 0x7FFFFFFF;
+struct WalkAcc {
+    float ax = 0.f, ay = 0.f, az = 0.f;
     unsigned approx = 0, exact = 0;
-    float acx = 0.f, acy = 0.f, acz = 0.f;
-    // FEATURES only: accepted-cell force, error-weighted direction tensor W,
-    // sum and minimum of (distance / cell edge)^2
     float ccx = 0.f, ccy = 0.f, ccz = 0.f;
     float wxx = 0.f, wyy = 0.f, wzz = 0.f, wxy = 0.f, wxz = 0.f, wyz = 0.f;
     float q2_sum = 0.f, q2_min = 3.0e38f;
-    int i = 0;
-    while (i < nodes) {
-        const NodeF32 cur = n32[__builtin_amdgcn_readfirstlane(i)];
-        const bool active = i >= skip_until;
-        if (cur.leaf >= 0) {
-            if (active && cur.leaf != t) {
-                const float dx = cur.cx - q.x;
-                const float dy = cur.cy - q.y;
-                const float dz = cur.cz - q.z;
-                const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
-                const float inv = __builtin_amdgcn_rsqf(r2);
-                const float s = (cur.m * inv) * (inv * inv);
-                acx = fmaf(s, dx, acx);
-                acy = fmaf(s, dy, acy);
-                acz = fmaf(s, dz, acz);
-                ++exact;
-            }
-            i = i + 1;
-        } else {
-            bool descend = false;
-            if (active) {
-                const float dx = cur.cx - q.x;
-                const float dy = cur.cy - q.y;
-                const float dz = cur.cz - q.z;
-                const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                if (d2 > 0.f && cur.size2 < theta2 * d2) {
-                    const float r2 = d2 + eps2;
-                    const float inv = __builtin_amdgcn_rsqf(r2);
-                    const float s = (cur.m * inv) * (inv * inv);
-                    acx = fmaf(s, dx, acx);
-                    acy = fmaf(s, dy, acy);
-                    acz = fmaf(s, dz, acz);
-                    ++approx;
-                    if (FEATURES) {
-                        ccx = fmaf(s, dx, ccx);
-                        ccy = fmaf(s, dy, ccy);
-                        ccz = fmaf(s, dz, ccz);
-                        // monopole truncation scale m size^2 / r^4 spread over
-                        // the direction d / r: w = m size^2 / r^6
-                        const float w = (s * cur.size2) * (inv * inv) * inv;
-                        const float wx = w * dx, wy = w * dy;
-                        wxx = fmaf(wx, dx, wxx);
-                        wyy = fmaf(wy, dy, wyy);
-                        wzz = fmaf(w * dz, dz, wzz);
-                        wxy = fmaf(wx, dy, wxy);
-                        wxz = fmaf(wx, dz, wxz);
-                        wyz = fmaf(wy, dz, wyz);
-                        const float q2 = d2 * cur.inv_size2;
-                        q2_sum += q2;
-                        q2_min = fminf(q2_min, q2);
-                    }
-                    skip_until = cur.skip;
-                } else {
-                    descend = true;
-                }
-            }
-            i = __builtin_amdgcn_ballot_w64(descend) != 0 ? i + 1 : cur.skip;
+
+    __device__ void leaf(float dx, float dy, float dz, float m, float eps2) {
+        const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
+        const float inv = __builtin_amdgcn_rsqf(r2);
+        const float s = (m * inv) * (inv * inv);
+        ax = fmaf(s, dx, ax);
+        ay = fmaf(s, dy, ay);
+        az = fmaf(s, dz, az);
+        ++exact;
+    }
+
+    // accepted cell at unsoftened squared distance d2
+    __device__ void cell(float dx, float dy, float dz, float d2, float m, float size2,
+                         float inv_size2, float eps2) {
+        const float r2 = d2 + eps2;
+        const float inv = __builtin_amdgcn_rsqf(r2);
+        const float s = (m * inv) * (inv * inv);
+        ax = fmaf(s, dx, ax);
+        ay = fmaf(s, dy, ay);
+        az = fmaf(s, dz, az);
+        ++approx;
+        if (FEATURES) {
+            ccx = fmaf(s, dx, ccx);
+            ccy = fmaf(s, dy, ccy);
+            ccz = fmaf(s, dz, ccz);
+            // monopole truncation scale m size^2 / r^4 spread over the
+            // direction d / r: w = m size^2 / r^6
+            const float w = (s * size2) * (inv * inv) * inv;
+            const float wx = w * dx, wy = w * dy;
+            wxx = fmaf(wx, dx, wxx);
+            wyy = fmaf(wy, dy, wyy);
+            wzz = fmaf(w * dz, dz, wzz);
+            wxy = fmaf(wx, dy, wxy);
+            wxz = fmaf(wx, dz, wxz);
+            wyz = fmaf(wy, dz, wyz);
+            const float q2 = d2 * inv_size2;
+            q2_sum += q2;
+            q2_min = fminf(q2_min, q2);
         }
     }
-    if (valid) {
+
+    // FEATURES: the kWalkRecord-float record of tree position t (G not
+    // applied; its consumer scatters G * sum). Otherwise the accelerations of
+    // particle tperm[t], in original order.
+    __device__ void store(int t, const uint32_t* __restrict__ tperm, float G,
+                          double* __restrict__ oax, double* __restrict__ oay,
+                          double* __restrict__ oaz, float* __restrict__ features) const {
         if (!FEATURES) {
-            // (with FEATURES the consumer of the record scatters G * sum itself)
             const uint32_t o = tperm[t];
-            ax[o] = static_cast<double>(G * acx);
-            ay[o] = static_cast<double>(G * acy);
-            az[o] = static_cast<double>(G * acz);
+            oax[o] = static_cast<double>(G * ax);
+            oay[o] = static_cast<double>(G * ay);
+            oaz[o] = static_cast<double>(G * az);
         } else {
-            // kWalkRecord floats per tree position (G not applied): total sum
-            // xyz, accepted cells; accepted-cell sum xyz, exact leaf terms;
-            // W xx yy zz xy; W xz yz, sum and minimum of (d / edge)^2
+            // total sum xyz, accepted cells; accepted-cell sum xyz, exact leaf
+            // terms; W xx yy zz xy; W xz yz, sum and minimum of (d / edge)^2
             float4* f = reinterpret_cast<float4*>(features + static_cast<size_t>(t) * kWalkRecord);
-            f[0] = make_float4(acx, acy, acz, static_cast<float>(approx));
+            f[0] = make_float4(ax, ay, az, static_cast<float>(approx));
             f[1] = make_float4(ccx, ccy, ccz, static_cast<float>(exact));
             f[2] = make_float4(wxx, wyy, wzz, wxy);
             f[3] = make_float4(wxz, wyz, q2_sum, approx > 0 ? q2_min : 0.f);
         }
     }
-    // wave totals: one atomic per wave
-    unsigned long long wa = valid ? approx : 0u, we = valid ? exact : 0u;
-    for (int off = 32; off > 0; off >>= 1) {
-        wa += __shfl_xor(wa, off);
-        we += __shfl_xor(we, off);
+};
+
+// One wave-step of the per-query walk at node `cur` (index i) for the query
+// at tree position t, position (qx, qy, qz): returns true when this query
+// descends into the node.
+template <bool FEATURES>
+__device__ inline bool walk_visit(const NodeF32& cur, int i, int t, float qx, float qy, float qz,
+                                  float eps2, float theta2, int& skip_until,
+                                  WalkAcc<FEATURES>& acc) {
+    if (i < skip_until) {
+        return false;
     }
-    if ((threadIdx.x & (kWave - 1)) == 0) {
-        atomicAdd(&counters[0], wa);
-        atomicAdd(&counters[1], we);
+    const float dx = cur.cx - qx;
+    const float dy = cur.cy - qy;
+    const float dz = cur.cz - qz;
+    if (cur.leaf >= 0) {
+        if (cur.leaf != t) {
+            acc.leaf(dx, dy, dz, cur.m, eps2);
+        }
+        return false;
     }
+    const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    if (d2 > 0.f && cur.size2 < theta2 * d2) {
+        acc.cell(dx, dy, dz, d2, cur.m, cur.size2, cur.inv_size2, eps2);
+        skip_until = cur.skip;
+        return false;
+    }
+    return true;
 }
 
-// Q queries per lane (tree positions base + 64k + lane, k < Q): a wave walks
-// the union of 64 Q consecutive queries' interaction lists, so every node
-// record fetched (the walk's latency chain) feeds Q times the arithmetic.
-// Each query keeps its own skip_until and its reference visit order.
-template <int Q>
-__global__ __launch_bounds__(kThreads) void walk_f32_multi_kernel(
+// FP32 walk with per-query reference interaction lists: one node per wave
+// step, node fields in SGPRs (scalar loads of a wave-uniform index); lanes with
+// nothing to do at a node are masked off. Each lane owns Q queries (tree
+// positions base + 64k + lane, k < Q), so a wave walks the union of 64 Q
+// consecutive queries' lists and every node record fetched (the walk's
+// latency chain) feeds Q times the arithmetic; each query keeps its own
+// skip_until and its reference visit order. (Measured alternatives, slower on
+// MI355X: branch-free steps, vector-loaded nodes, prefetching both
+// successors, four queries per lane.)
+template <bool FEATURES, int Q>
+__global__ __launch_bounds__(kThreads) void walk_f32_kernel(
     const NodeF32* __restrict__ n32, int nodes, const float4* __restrict__ s4,
     const uint32_t* __restrict__ tperm, int n, float G, float eps2, float theta2,
     double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az,
-    unsigned long long* __restrict__ counters) {
+    unsigned long long* __restrict__ counters, float* __restrict__ features) {
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = xcd_block(blockIdx.x, gridDim.x) * (kThreads / kWave) + threadIdx.x / kWave;
     const int base = wave * (kWave * Q);
     int t[Q], skip_until[Q];
-    float qx[Q], qy[Q], qz[Q], acx[Q], acy[Q], acz[Q];
-    unsigned approx[Q], exact[Q];
+    float qx[Q], qy[Q], qz[Q];
+    WalkAcc<FEATURES> acc[Q];
 #pragma unroll
     for (int k = 0; k < Q; ++k) {
         t[k] = base + k * kWave + lane;
@@ -924,65 +929,270 @@ __global__ __launch_bounds__(kThreads) void walk_f32_multi_kernel(
         qy[k] = q.y;
         qz[k] = q.z;
         skip_until[k] = valid ? 0 : 0x7FFFFFFF;
-        acx[k] = acy[k] = acz[k] = 0.f;
-        approx[k] = exact[k] = 0u;
     }
     int i = 0;
     while (i < nodes) {
         const NodeF32 cur = n32[__builtin_amdgcn_readfirstlane(i)];
-        if (cur.leaf >= 0) {
+        bool descend = false;
 #pragma unroll
-            for (int k = 0; k < Q; ++k) {
-                if (i >= skip_until[k] && cur.leaf != t[k]) {
-                    const float dx = cur.cx - qx[k];
-                    const float dy = cur.cy - qy[k];
-                    const float dz = cur.cz - qz[k];
-                    const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
-                    const float inv = __builtin_amdgcn_rsqf(r2);
-                    const float s = (cur.m * inv) * (inv * inv);
-                    acx[k] = fmaf(s, dx, acx[k]);
-                    acy[k] = fmaf(s, dy, acy[k]);
-                    acz[k] = fmaf(s, dz, acz[k]);
-                    ++exact[k];
-                }
-            }
-            i = i + 1;
-        } else {
+        for (int k = 0; k < Q; ++k) {
+            descend |= walk_visit<FEATURES>(cur, i, t[k], qx[k], qy[k], qz[k], eps2, theta2,
+                                            skip_until[k], acc[k]);
+        }
+        i = cur.leaf >= 0 || __builtin_amdgcn_ballot_w64(descend) != 0 ? i + 1 : cur.skip;
+    }
+    unsigned long long wa = 0, we = 0;
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        if (t[k] < n) {
+            wa += acc[k].approx;
+            we += acc[k].exact;
+            acc[k].store(t[k], tperm, G, ax, ay, az, features);
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        wa += __shfl_xor(wa, off);
+        we += __shfl_xor(we, off);
+    }
+    if (lane == 0) {
+        atomicAdd(&counters[0], wa);
+        atomicAdd(&counters[1], we);
+    }
+}
+
+// Group walk (FP32). A wave takes 64 Q consecutive tree positions (Q per lane) as one group
+// and walks the tree once for the group's bounding box: a cell is accepted
+// when the reference's test (d^2 > 0 and 4h^2 < theta^2 d^2) holds for the
+// box distance d, which implies it holds for every member, so each member's
+// interaction set refines its reference set. The pre-order node array is
+// scanned 64 nodes per step (one vector load per lane); node j of the window
+// is visited unless an earlier closed node k of the window (accepted cell or
+// leaf) has skip[k] > j, found with a wave prefix-max of skip over closed
+// nodes; the next window starts at max(i + 64, that maximum). Visited closed
+// nodes are staged in LDS and evaluated by all 64 lanes without divergence.
+// A group that is not compact (64 consecutive positions of the Morton curve
+// can straddle one of its jumps) would get a box with most of the tree in
+// its near field; such a wave walks per query with reference lists instead.
+constexpr int kGroupList = 128;      // staged cells and staged leaves per wave (each)
+constexpr int kGroupCell = 1024;     // compact: the group's smallest enclosing cell holds <= 1024 Q
+// Queries per lane of the group walk: one below kGroupPairFrom particles, two
+// from there on (measured on MI355X: at 1M, 1 is ~8% faster; at 10M, 2 is
+// ~20% faster, the wider box costing less than the shared traversal saves).
+constexpr size_t kGroupPairFrom = 4u << 20;
+
+// Exclusive prefix max over the wave of non-negative ints, with DPP row
+// shifts and row broadcasts (no LDS round trips); `total` = max over the wave.
+__device__ inline int dpp_max(int v, int shifted) {
+    return v > shifted ? v : shifted;
+}
+
+__device__ inline int wave_prefix_max_exclusive(int v, int lane, int& total) {
+    // inclusive scan within each 16-lane row (out-of-row sources read 0)
+    v = dpp_max(v, __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false));   // row_shr:1
+    v = dpp_max(v, __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false));   // row_shr:2
+    v = dpp_max(v, __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false));   // row_shr:4
+    v = dpp_max(v, __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false));   // row_shr:8
+    // across rows: lane 15 of each row into the next row, lane 31 into rows 2, 3
+    v = dpp_max(v, __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false));   // row_bcast:15
+    v = dpp_max(v, __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false));   // row_bcast:31
+    total = __builtin_amdgcn_readlane(v, kWave - 1);
+    const int e = __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, false);      // wave_shr:1
+    return lane == 0 ? 0 : e;
+}
+
+__device__ inline float wave_min(float v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        v = fminf(v, __shfl_xor(v, off));
+    }
+    return v;
+}
+
+__device__ inline float wave_max(float v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        v = fmaxf(v, __shfl_xor(v, off));
+    }
+    return v;
+}
+
+// Orders one wave's LDS writes before its later LDS reads.
+__device__ inline void wave_lds_fence() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <bool FEATURES, int Q>
+__global__ __launch_bounds__(kThreads) void group_walk_kernel(
+    const NodeF32* __restrict__ n32, int nodes, const float4* __restrict__ s4,
+    const uint32_t* __restrict__ tperm, const uint64_t* __restrict__ tword,
+    const uint64_t* __restrict__ ext, int n, float G, float eps2, float theta2,
+    double* __restrict__ ax, double* __restrict__ ay,
+    double* __restrict__ az, unsigned long long* __restrict__ counters,
+    float* __restrict__ features) {
+    // staged lists of one wave: accepted cells (COM, mass; size^2 and
+    // 1/size^2 with FEATURES) and leaves (position, mass; tree position)
+    __shared__ float4 lcell_all[kThreads / kWave][kGroupList];
+    __shared__ float4 lleaf_all[kThreads / kWave][kGroupList];
+    __shared__ int ltag_all[kThreads / kWave][kGroupList];
+    __shared__ float2 lsz_all[FEATURES ? kThreads / kWave : 1][FEATURES ? kGroupList : 1];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    float4* lcell = lcell_all[wid];
+    float4* lleaf = lleaf_all[wid];
+    int* ltag = ltag_all[wid];
+    float2* lsz = lsz_all[FEATURES ? wid : 0];
+    const int wave = xcd_block(blockIdx.x, gridDim.x) * (kThreads / kWave) + wid;
+    const int t0 = wave * kWave * Q;
+    if (t0 >= n) {
+        return;   // whole wave beyond the particles
+    }
+    // the group: tree positions t0 + 64k + lane, k < Q
+    int t[Q];
+    float qx[Q], qy[Q], qz[Q];
+    float bx0 = 3.0e38f, by0 = 3.0e38f, bz0 = 3.0e38f;
+    float bx1 = -3.0e38f, by1 = -3.0e38f, bz1 = -3.0e38f;
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        t[k] = t0 + k * kWave + lane;
+        const bool valid = t[k] < n;
+        const float4 q = s4[valid ? t[k] : n - 1];
+        qx[k] = q.x;
+        qy[k] = q.y;
+        qz[k] = q.z;
+        if (valid) {
+            bx0 = fminf(bx0, q.x);
+            by0 = fminf(by0, q.y);
+            bz0 = fminf(bz0, q.z);
+            bx1 = fmaxf(bx1, q.x);
+            by1 = fmaxf(by1, q.y);
+            bz1 = fmaxf(bz1, q.z);
+        }
+    }
+    bx0 = wave_min(bx0);
+    by0 = wave_min(by0);
+    bz0 = wave_min(bz0);
+    bx1 = wave_max(bx1);
+    by1 = wave_max(by1);
+    bz1 = wave_max(bz1);
+    // Compactness: the deepest cell holding the whole group (common key
+    // prefix of its first and last positions, as tree order is key order)
+    // bounds the box, and that cell's particle count bounds what the box
+    // can contain; a group whose cell holds more than kGroupCell Q particles
+    // (it straddles a jump of the Morton curve, an outlier, a dense core
+    // and its outskirts) walks per query instead.
+    const int t1 = t0 + kWave * Q - 1 < n ? t0 + kWave * Q - 1 : n - 1;
+    const int c = common_levels(tword, ext, t0, t1);
+    const int cell_count = run_end(tword, ext, n, t1, c) - run_start(tword, ext, t0, c) + 1;
+    const bool compact = cell_count <= kGroupCell * Q;
+    WalkAcc<FEATURES> acc[Q];
+    if (!compact) {
+        int skip_until[Q];
+#pragma unroll
+        for (int k = 0; k < Q; ++k) {
+            skip_until[k] = t[k] < n ? 0 : 0x7FFFFFFF;
+        }
+        int i = 0;
+        while (i < nodes) {
+            const NodeF32 cur = n32[__builtin_amdgcn_readfirstlane(i)];
             bool descend = false;
 #pragma unroll
             for (int k = 0; k < Q; ++k) {
-                if (i >= skip_until[k]) {
-                    const float dx = cur.cx - qx[k];
-                    const float dy = cur.cy - qy[k];
-                    const float dz = cur.cz - qz[k];
-                    const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                    if (d2 > 0.f && cur.size2 < theta2 * d2) {
-                        const float r2 = d2 + eps2;
-                        const float inv = __builtin_amdgcn_rsqf(r2);
-                        const float s = (cur.m * inv) * (inv * inv);
-                        acx[k] = fmaf(s, dx, acx[k]);
-                        acy[k] = fmaf(s, dy, acy[k]);
-                        acz[k] = fmaf(s, dz, acz[k]);
-                        ++approx[k];
-                        skip_until[k] = cur.skip;
+                descend |= walk_visit<FEATURES>(cur, i, t[k], qx[k], qy[k], qz[k], eps2, theta2,
+                                                skip_until[k], acc[k]);
+            }
+            i = cur.leaf >= 0 || __builtin_amdgcn_ballot_w64(descend) != 0 ? i + 1 : cur.skip;
+        }
+    } else {
+        const uint64_t lt = lane == 0 ? 0ull : ((~0ull) >> (kWave - lane));
+        int ccnt = 0, lcnt = 0;   // staged cells, leaves (wave-uniform)
+        int i = 0;                // window start (wave-uniform)
+        while (true) {
+            const bool more = i < nodes;
+            if (more) {
+                const int j = i + lane;
+                const bool in = j < nodes;
+                NodeF32 nd;
+                nd.leaf = -1;
+                nd.skip = 0;
+                bool closed = false, leaf = false;
+                if (in) {
+                    nd = n32[j];
+                    leaf = nd.leaf >= 0;
+                    if (leaf) {
+                        closed = true;
                     } else {
-                        descend = true;
+                        const float ddx = fmaxf(0.f, fmaxf(bx0 - nd.cx, nd.cx - bx1));
+                        const float ddy = fmaxf(0.f, fmaxf(by0 - nd.cy, nd.cy - by1));
+                        const float ddz = fmaxf(0.f, fmaxf(bz0 - nd.cz, nd.cz - bz1));
+                        const float d2 = fmaf(ddx, ddx, fmaf(ddy, ddy, ddz * ddz));
+                        closed = d2 > 0.f && nd.size2 < theta2 * d2;
                     }
                 }
+                int window_max;
+                const int before = wave_prefix_max_exclusive(closed ? nd.skip : 0, lane, window_max);
+                const bool take = closed && before <= j;
+                const uint64_t cbal = __ballot(take && !leaf);
+                const uint64_t lbal = __ballot(take && leaf);
+                if (take && !leaf) {
+                    const int pos = ccnt + static_cast<int>(__popcll(cbal & lt));
+                    lcell[pos] = make_float4(nd.cx, nd.cy, nd.cz, nd.m);
+                    if (FEATURES) {
+                        lsz[pos] = make_float2(nd.size2, nd.inv_size2);
+                    }
+                }
+                if (take && leaf) {
+                    const int pos = lcnt + static_cast<int>(__popcll(lbal & lt));
+                    lleaf[pos] = make_float4(nd.cx, nd.cy, nd.cz, nd.m);
+                    ltag[pos] = nd.leaf;
+                }
+                ccnt += static_cast<int>(__popcll(cbal));
+                lcnt += static_cast<int>(__popcll(lbal));
+                i = window_max > i + kWave ? window_max : i + kWave;
             }
-            i = __builtin_amdgcn_ballot_w64(descend) != 0 ? i + 1 : cur.skip;
+            if (ccnt > kGroupList - kWave || lcnt > kGroupList - kWave ||
+                (!more && ccnt + lcnt > 0)) {
+                // evaluate the staged lists for every member
+                wave_lds_fence();
+                for (int e = 0; e < ccnt; ++e) {
+                    const float4 p = lcell[e];
+                    const float2 sz = FEATURES ? lsz[e] : make_float2(0.f, 0.f);
+#pragma unroll
+                    for (int k = 0; k < Q; ++k) {
+                        const float dx = p.x - qx[k];
+                        const float dy = p.y - qy[k];
+                        const float dz = p.z - qz[k];
+                        acc[k].cell(dx, dy, dz, fmaf(dx, dx, fmaf(dy, dy, dz * dz)), p.w, sz.x,
+                                    sz.y, eps2);
+                    }
+                }
+                for (int e = 0; e < lcnt; ++e) {
+                    const float4 p = lleaf[e];
+                    const int tag = ltag[e];
+#pragma unroll
+                    for (int k = 0; k < Q; ++k) {
+                        if (tag != t[k]) {
+                            acc[k].leaf(p.x - qx[k], p.y - qy[k], p.z - qz[k], p.w, eps2);
+                        }
+                    }
+                }
+                ccnt = 0;
+                lcnt = 0;
+                wave_lds_fence();
+            }
+            if (!more) {
+                break;
+            }
         }
     }
     unsigned long long wa = 0, we = 0;
 #pragma unroll
     for (int k = 0; k < Q; ++k) {
         if (t[k] < n) {
-            const uint32_t o = tperm[t[k]];
-            ax[o] = static_cast<double>(G * acx[k]);
-            ay[o] = static_cast<double>(G * acy[k]);
-            az[o] = static_cast<double>(G * acz[k]);
-            wa += approx[k];
-            we += exact[k];
+            wa += acc[k].approx;
+            we += acc[k].exact;
+            acc[k].store(t[k], tperm, G, ax, ay, az, features);
         }
     }
     for (int off = 32; off > 0; off >>= 1) {
@@ -1215,6 +1425,7 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error)
     NBX_TRY(s.n32.need<NodeF32>(nodes, error));
     NodeSoA nd = node_soa(s);
     depth_offsets_kernel<<<1, 1, 0, st>>>(flags + kDCount, flags + kDOff);
+    s.ext_tree = ext_tree;
     fill_kernel<<<g, kThreads, 0, st>>>(B<uint64_t>(s.tword), ext_tree, B<int32_t>(s.delta),
                                         B<uint32_t>(s.cnt), B<uint32_t>(s.scan), ni,
                                         B<double>(s.sx), B<double>(s.sy), B<double>(s.sz),
@@ -1237,7 +1448,7 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error)
             com_level_kernel<<<grid_for(count, kThreads), kThreads, 0, st>>>(
                 B<int>(s.nlist) + offs[d], static_cast<int>(count), B<int>(s.ninode),
                 B<int>(s.nkids), nd, B<uint32_t>(s.trank),
-                B<double>(s.sx), B<double>(s.sy), B<double>(s.sz), b4, s.root[3],
+                B<double>(s.sx), B<double>(s.sy), B<double>(s.sz), B<double>(s.sm), b4, s.root[3],
                 n64_out, B<NodeF32>(s.n32));
             --d;
             continue;
@@ -1249,8 +1460,8 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error)
         com_small_levels_kernel<<<1, 1024, 0, st>>>(
             B<int>(s.nlist), flags + kDOff, flags + kDCount, d, lo, B<int>(s.ninode),
             B<int>(s.nkids), nd, B<uint32_t>(s.trank),
-            B<double>(s.sx), B<double>(s.sy), B<double>(s.sz), b4, s.root[3], n64_out,
-            B<NodeF32>(s.n32));
+            B<double>(s.sx), B<double>(s.sy), B<double>(s.sz), B<double>(s.sm), b4, s.root[3],
+            n64_out, B<NodeF32>(s.n32));
         d = lo - 1;
     }
     NBX_HIP(error, hipGetLastError());
@@ -1279,7 +1490,6 @@ NBodySimProStatus bh_walk(Engine* e, const NBodySimProGravity* g, bool fp64_exac
     NBX_HIP(error, hipEventRecord(e->ev[5], st));
     const double theta2 = e->theta * e->theta;
     const unsigned blocks = grid_for(s.n, kThreads);
-    const unsigned grid = (blocks + kXcds - 1) / kXcds * kXcds;
     if (fp64_exact && !s.has_n64) {
         NBX_FAIL(error, N_BODY_SIM_PRO_STATUS_INVALID_STATE,
                  "the exact walk needs a build on FP64 inputs");
@@ -1289,26 +1499,54 @@ NBodySimProStatus bh_walk(Engine* e, const NBodySimProGravity* g, bool fp64_exac
             B<NodeF64>(s.n64), static_cast<int>(s.nodes), B<double>(s.sx), B<double>(s.sy),
             B<double>(s.sz), B<uint32_t>(s.tperm), ni, g->gravitational_constant,
             g->softening_squared, theta2, e->ax, e->ay, e->az, B<unsigned long long>(s.counters));
-    } else if (features != nullptr) {
-        walk_f32_kernel<true><<<grid, kThreads, 0, st>>>(
-            B<NodeF32>(s.n32), static_cast<int>(s.nodes), B<float4>(s.s4), B<uint32_t>(s.tperm),
-            ni, static_cast<float>(g->gravitational_constant),
-            static_cast<float>(g->softening_squared), static_cast<float>(theta2), e->ax, e->ay,
-            e->az, B<unsigned long long>(s.counters), features);
-    } else if (kWalkQueries == 1) {
-        walk_f32_kernel<false><<<grid, kThreads, 0, st>>>(
-            B<NodeF32>(s.n32), static_cast<int>(s.nodes), B<float4>(s.s4), B<uint32_t>(s.tperm),
-            ni, static_cast<float>(g->gravitational_constant),
-            static_cast<float>(g->softening_squared), static_cast<float>(theta2), e->ax, e->ay,
-            e->az, B<unsigned long long>(s.counters), nullptr);
+    } else if (!(e->flags & NBX_FLAG_BH_REFERENCE_LISTS)) {
+        const NodeF32* nodes32 = B<NodeF32>(s.n32);
+        const int nn = static_cast<int>(s.nodes);
+        const float Gf = static_cast<float>(g->gravitational_constant);
+        const float e2 = static_cast<float>(g->softening_squared);
+        const float th2 = static_cast<float>(theta2);
+        unsigned long long* cnts = B<unsigned long long>(s.counters);
+        auto grid_q = [&](int q) {
+            const unsigned blocks_q = grid_for(s.n, kThreads * q);
+            return (blocks_q + kXcds - 1) / kXcds * kXcds;
+        };
+        if (s.n < kGroupPairFrom) {
+            if (features != nullptr) {
+                group_walk_kernel<true, 1><<<grid_q(1), kThreads, 0, st>>>(
+                    nodes32, nn, B<float4>(s.s4), B<uint32_t>(s.tperm), B<uint64_t>(s.tword),
+                    s.ext_tree, ni, Gf, e2, th2, e->ax, e->ay, e->az, cnts, features);
+            } else {
+                group_walk_kernel<false, 1><<<grid_q(1), kThreads, 0, st>>>(
+                    nodes32, nn, B<float4>(s.s4), B<uint32_t>(s.tperm), B<uint64_t>(s.tword),
+                    s.ext_tree, ni, Gf, e2, th2, e->ax, e->ay, e->az, cnts, nullptr);
+            }
+        } else {
+            if (features != nullptr) {
+                group_walk_kernel<true, 2><<<grid_q(2), kThreads, 0, st>>>(
+                    nodes32, nn, B<float4>(s.s4), B<uint32_t>(s.tperm), B<uint64_t>(s.tword),
+                    s.ext_tree, ni, Gf, e2, th2, e->ax, e->ay, e->az, cnts, features);
+            } else {
+                group_walk_kernel<false, 2><<<grid_q(2), kThreads, 0, st>>>(
+                    nodes32, nn, B<float4>(s.s4), B<uint32_t>(s.tperm), B<uint64_t>(s.tword),
+                    s.ext_tree, ni, Gf, e2, th2, e->ax, e->ay, e->az, cnts, nullptr);
+            }
+        }
     } else {
-        const unsigned mblocks = grid_for(s.n, kThreads * kWalkQueries);
-        const unsigned mgrid = (mblocks + kXcds - 1) / kXcds * kXcds;
-        walk_f32_multi_kernel<kWalkQueries><<<mgrid, kThreads, 0, st>>>(
-            B<NodeF32>(s.n32), static_cast<int>(s.nodes), B<float4>(s.s4), B<uint32_t>(s.tperm),
-            ni, static_cast<float>(g->gravitational_constant),
-            static_cast<float>(g->softening_squared), static_cast<float>(theta2), e->ax, e->ay,
-            e->az, B<unsigned long long>(s.counters));
+        const unsigned wblocks = grid_for(s.n, kThreads * kWalkQueries);
+        const unsigned wgrid = (wblocks + kXcds - 1) / kXcds * kXcds;
+        if (features != nullptr) {
+            walk_f32_kernel<true, kWalkQueries><<<wgrid, kThreads, 0, st>>>(
+                B<NodeF32>(s.n32), static_cast<int>(s.nodes), B<float4>(s.s4),
+                B<uint32_t>(s.tperm), ni, static_cast<float>(g->gravitational_constant),
+                static_cast<float>(g->softening_squared), static_cast<float>(theta2), e->ax,
+                e->ay, e->az, B<unsigned long long>(s.counters), features);
+        } else {
+            walk_f32_kernel<false, kWalkQueries><<<wgrid, kThreads, 0, st>>>(
+                B<NodeF32>(s.n32), static_cast<int>(s.nodes), B<float4>(s.s4),
+                B<uint32_t>(s.tperm), ni, static_cast<float>(g->gravitational_constant),
+                static_cast<float>(g->softening_squared), static_cast<float>(theta2), e->ax,
+                e->ay, e->az, B<unsigned long long>(s.counters), nullptr);
+        }
     }
     NBX_HIP(error, hipGetLastError());
     NBX_HIP(error, hipEventRecord(e->ev[6], st));

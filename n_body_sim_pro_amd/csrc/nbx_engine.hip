@@ -324,6 +324,7 @@ NBXEngine* nbx_engine_create(const NBXConfig* config, NBodySimProError* error) {
     e->device = config->device;
     e->capacity = config->capacity;
     e->theta = config->theta > 0.0 ? config->theta : 0.7;
+    e->flags = config->flags;
     bool ok = hipSetDevice(e->device) == hipSuccess &&
               hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) == hipSuccess;
     double** arrays[] = {&e->px, &e->py, &e->pz, &e->vx, &e->vy, &e->vz,
@@ -650,6 +651,30 @@ NBodySimProStatus nbx_engine_bh_export_nodes(NBXEngine* engine, double* rows,
     }
     NBX_HIP(error, hipMemcpy(word.data(), s.tword.p, n * 8, hipMemcpyDeviceToHost));
     NBX_HIP(error, hipMemcpy(trank.data(), s.trank.p, n * 4, hipMemcpyDeviceToHost));
+    // The node SoA holds internal nodes only; a leaf's row comes from its
+    // tree position t: depth one below the deeper neighbour cell, the
+    // particle itself as mass and centre of mass.
+    std::vector<int32_t> delta(n);
+    std::vector<double> sx(n), sy(n), sz(n), sm(n);
+    NBX_HIP(error, hipMemcpy(delta.data(), s.delta.p, n * 4, hipMemcpyDeviceToHost));
+    NBX_HIP(error, hipMemcpy(sx.data(), s.sx.p, n * 8, hipMemcpyDeviceToHost));
+    NBX_HIP(error, hipMemcpy(sy.data(), s.sy.p, n * 8, hipMemcpyDeviceToHost));
+    NBX_HIP(error, hipMemcpy(sz.data(), s.sz.p, n * 8, hipMemcpyDeviceToHost));
+    NBX_HIP(error, hipMemcpy(sm.data(), s.sm.p, n * 8, hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < nodes; ++k) {
+        const int t = leaf[k];
+        if (t < 0) {
+            continue;
+        }
+        const int a = t > 0 ? delta[t - 1] : -1;
+        depth[k] = 1 + (a > delta[t] ? a : delta[t]);
+        start[k] = t;
+        count[k] = 1;
+        m[k] = sm[t];
+        cx[k] = sx[t];
+        cy[k] = sy[t];
+        cz[k] = sz[t];
+    }
     if (s.tie_path) {
         ext.resize(n * 6);
         NBX_HIP(error, hipMemcpy(ext.data(), static_cast<uint64_t*>(s.ext.p) + 2 * n * 6,

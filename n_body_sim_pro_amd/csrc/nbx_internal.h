@@ -92,6 +92,7 @@ struct BHState {
     bool tie_path = false;
     bool mismatch_path = false;
     bool has_n64 = false;     // the last build wrote FP64 node records
+    const uint64_t* ext_tree = nullptr;   // tree-order extension words (tie path) or null
     double root[4] = {0, 0, 0, 1};
     // build inputs (FP64 x, y, z, m): either the master arrays or FP32-rounded copies
     Buf b4;                   // (x, y, z, m) per particle in original order
@@ -126,6 +127,7 @@ struct Engine {
     size_t capacity = 0;
     size_t n = 0;
     double theta = 0.7;
+    uint32_t flags = 0;       // NBXConfig.flags
     // FP64 master state (SoA)
     double *px = nullptr, *py = nullptr, *pz = nullptr;
     double *vx = nullptr, *vy = nullptr, *vz = nullptr;

@@ -629,9 +629,17 @@ __device__ inline void raw_features(const float* __restrict__ wf, const Frame& f
     }
 }
 
+// Depth of the leaf at tree position t: one below the deeper of the cells it
+// shares with its neighbours (delta[t] = common levels of t and t + 1, -1 at
+// the ends).
+__device__ inline int leaf_depth(const int32_t* __restrict__ delta, int t) {
+    const int a = t > 0 ? delta[t - 1] : -1;
+    const int b = delta[t];
+    return 1 + (a > b ? a : b);
+}
+
 __global__ void features_kernel(const float* __restrict__ walkf, const float4* __restrict__ s4,
-                                const int32_t* __restrict__ ndepth,
-                                const uint32_t* __restrict__ scan, const uint32_t* __restrict__ cnt,
+                                const int32_t* __restrict__ delta,
                                 const uint32_t* __restrict__ tperm, int n, float log2_root_edge,
                                 float G, double* __restrict__ ax, double* __restrict__ ay,
                                 double* __restrict__ az, float* __restrict__ out,
@@ -641,11 +649,10 @@ __global__ void features_kernel(const float* __restrict__ walkf, const float4* _
         return;
     }
     float f[kF];
-    const int li = static_cast<int>(scan[t] + cnt[t]) + t;
     const float* wf = walkf + static_cast<size_t>(t) * kWalkRecord;
     Frame fr;
     local_frame(wf, fr);
-    raw_features(wf, fr, s4[t].w, ndepth[li], log2_root_edge, f);
+    raw_features(wf, fr, s4[t].w, leaf_depth(delta, t), log2_root_edge, f);
     const size_t p = tperm[t];
     // the plain walk's accelerations, bit for bit
     ax[p] = static_cast<double>(G * wf[0]);
@@ -674,8 +681,7 @@ __global__ void features_kernel(const float* __restrict__ walkf, const float4* _
 __global__ __launch_bounds__(kMlpThreads) void learned_apply_kernel(
     const float* __restrict__ P, const float* __restrict__ mean, const float* __restrict__ inv_std,
     float out_scale, const float* __restrict__ walkf, const float4* __restrict__ s4,
-    const int32_t* __restrict__ ndepth, const uint32_t* __restrict__ scan,
-    const uint32_t* __restrict__ cnt, const uint32_t* __restrict__ tperm, int n,
+    const int32_t* __restrict__ delta, const uint32_t* __restrict__ tperm, int n,
     float log2_root_edge, float G, double* __restrict__ ax, double* __restrict__ ay,
     double* __restrict__ az) {
     __shared__ float xs_all[kWavesPerBlock][kWave][kF + 1];
@@ -696,10 +702,9 @@ __global__ __launch_bounds__(kMlpThreads) void learned_apply_kernel(
         float f[kF];
         double bx = 0.0, by = 0.0, bz = 0.0;
         if (valid) {
-            const int li = static_cast<int>(scan[t] + cnt[t]) + t;
             const float* wf = walkf + static_cast<size_t>(t) * kWalkRecord;
             local_frame(wf, fr);
-            raw_features(wf, fr, s4[t].w, ndepth[li], log2_root_edge, f);
+            raw_features(wf, fr, s4[t].w, leaf_depth(delta, t), log2_root_edge, f);
             // the plain walk's accelerations, bit for bit
             bx = static_cast<double>(G * wf[0]);
             by = static_cast<double>(G * wf[1]);
@@ -755,8 +760,7 @@ NBodySimProStatus bh_learned(Engine* e, const NBodySimProGravity* g, NBodySimPro
     blocks = blocks < m->cu_count * 4 ? blocks : m->cu_count * 4;
     learned_apply_kernel<<<blocks, kMlpThreads, 0, e->stream>>>(
         m->params, m->norm, m->norm + 2 * kF, m->out_scale, walkf, s.s4.as<float4>(),
-        s.ndepth.as<int32_t>(), s.scan.as<uint32_t>(), s.cnt.as<uint32_t>(),
-        s.tperm.as<uint32_t>(), static_cast<int>(s.n),
+        s.delta.as<int32_t>(), s.tperm.as<uint32_t>(), static_cast<int>(s.n),
         static_cast<float>(std::log2(2.0 * s.root[3])),
         static_cast<float>(g->gravitational_constant), e->ax, e->ay, e->az);
     NBX_HIP(error, hipGetLastError());
@@ -1059,8 +1063,8 @@ NBodySimProStatus nbx_engine_bh_features(NBXEngine* engine, const NBodySimProGra
     float* dfeat = e->scratch.as<float>();
     float* dframe = dfeat + s.n * nbx::kF;
     nbx::features_kernel<<<nbx::grid_for(s.n, 256), 256, 0, e->stream>>>(
-        e->scratch2.as<float>(), s.s4.as<float4>(), s.ndepth.as<int32_t>(), s.scan.as<uint32_t>(),
-        s.cnt.as<uint32_t>(), s.tperm.as<uint32_t>(), static_cast<int>(s.n),
+        e->scratch2.as<float>(), s.s4.as<float4>(), s.delta.as<int32_t>(),
+        s.tperm.as<uint32_t>(), static_cast<int>(s.n),
         static_cast<float>(std::log2(2.0 * s.root[3])),
         static_cast<float>(g->gravitational_constant), e->ax, e->ay, e->az, dfeat,
         frames != nullptr ? dframe : nullptr);

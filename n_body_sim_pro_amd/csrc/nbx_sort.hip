@@ -6,7 +6,7 @@
 // (src/c/barnes_hut/barnes_hut.c:312-339). Any stable sort on the full key
 // yields the same permutation (ties keep ascending original index). Here the
 // digit is 9 bits (7 passes cover 63 bits) and a pass is three launches:
-//   sort_histogram   per-tile digit counts (4096-item tiles, LDS atomics);
+//   sort_histogram   per-tile digit counts (2048-item tiles, LDS atomics);
 //   sort_scan_digit  per digit, exclusive scan over tiles (+ digit totals);
 //   sort_scatter     each tile ranks its items stably in LDS (wave ballots
 //                    match equal digits; waves and rounds keep input order),
@@ -18,13 +18,14 @@
 namespace nbx {
 namespace {
 
-constexpr int kSortThreads = 256;
+constexpr int kSortThreads = 512;
 constexpr int kSortWaves = kSortThreads / kWave;
-constexpr int kSortIpt = 16;
-constexpr int kSortTile = kSortThreads * kSortIpt;   // 4096
+constexpr int kSortIpt = 4;
+constexpr int kSortTile = kSortThreads * kSortIpt;   // 2048
 constexpr int kDigitBits = 9;
 constexpr int kRadix = 1 << kDigitBits;             // 512
 constexpr int kDigitsPerThread = kRadix / kSortThreads;
+static_assert(kRadix % kSortThreads == 0, "each thread owns whole digits");
 
 __device__ inline unsigned digit_of(uint64_t key, int shift) {
     return static_cast<unsigned>((key >> shift) & (kRadix - 1));
@@ -41,7 +42,7 @@ __global__ __launch_bounds__(kSortThreads) void sort_histogram(const uint64_t* _
     }
     __syncthreads();
     const size_t base = static_cast<size_t>(tile) * kSortTile;
-#pragma unroll 4
+#pragma unroll
     for (int r = 0; r < kSortIpt; ++r) {
         const size_t i = base + static_cast<size_t>(r) * kSortThreads + threadIdx.x;
         if (i < n) {
@@ -66,9 +67,9 @@ __device__ inline uint32_t wave_inclusive_scan(uint32_t v, unsigned lane) {
     return v;
 }
 
-// Exclusive scan over the block's 256 threads; `total` gets the block sum.
+// Exclusive scan over the block's kSortThreads threads; `total` gets the block sum.
 // `wsum` holds kSortWaves entries; the caller syncs before reusing it.
-__device__ inline uint32_t block_exclusive_scan256(uint32_t v, uint32_t* wsum, uint32_t& total) {
+__device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* wsum, uint32_t& total) {
     const unsigned lane = threadIdx.x & (kWave - 1);
     const unsigned wave = threadIdx.x / kWave;
     const uint32_t incl = wave_inclusive_scan(v, lane);
@@ -97,7 +98,7 @@ __global__ __launch_bounds__(kSortThreads) void sort_scan_digit(uint32_t* __rest
         const unsigned i = start + threadIdx.x;
         const uint32_t v = i < tiles ? row[i] : 0u;
         uint32_t total;
-        const uint32_t ex = block_exclusive_scan256(v, wsum, total);
+        const uint32_t ex = block_exclusive_scan(v, wsum, total);
         if (i < tiles) {
             row[i] = carry + ex;
         }
@@ -141,9 +142,9 @@ __global__ __launch_bounds__(kSortThreads) void sort_scatter(
         csum += cnt[j];
     }
     uint32_t total;
-    uint32_t gstart = block_exclusive_scan256(tsum, wsum, total);
+    uint32_t gstart = block_exclusive_scan(tsum, wsum, total);
     __syncthreads();
-    uint32_t lstart = block_exclusive_scan256(csum, wsum, total);
+    uint32_t lstart = block_exclusive_scan(csum, wsum, total);
 #pragma unroll
     for (int j = 0; j < kDigitsPerThread; ++j) {
         const unsigned d = tid * kDigitsPerThread + j;

@@ -9,7 +9,7 @@ src/c/barnes_hut/barnes_hut.c:611), so this restates the network that
 include/nbx_learned.h specifies, with the same rounding points as the HIP
 kernels:
 
-    xb  = bf16((f - mean) / std)
+    xb  = bf16((f - mean) * inv_std),  inv_std = float32(1 / std)
     z1  = W1b xb + b1            h1b = bf16(silu(z1))
     z2  = W2b h1b + b2           h2b = bf16(silu(z2))
     y   = W3b h2b + b3           (outputs 0..2 used)
@@ -107,7 +107,9 @@ def forward(params, mean, std, feats, rounding=True):
     P = unpack(params)
     dt = np.float32 if rounding else np.float64
     f = np.asarray(feats, dt)
-    xs = ((f - np.asarray(mean, dt)) / np.asarray(std, dt)).T          # F x n
+    with np.errstate(divide="ignore"):
+        inv_std = (np.float32(1.0) / np.asarray(std, np.float32)).astype(dt)   # +inf -> 0
+    xs = ((f - np.asarray(mean, dt)) * inv_std).T                             # F x n
     xb = _round(xs, rounding)
     W1, W2, W3 = (_round(P[k], rounding) for k in ("W1", "W2", "W3"))
     z1 = _mm(W1, xb, rounding) + P["b1"][:, None].astype(dt)

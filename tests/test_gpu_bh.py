@@ -1,7 +1,9 @@
 """Barnes-Hut parity on the GPU: Morton keys, sorted order, root cube and tree
 topology bit-exact against the oracle; hip_barnes_hut_fp64 accelerations and
 statistics bit-identical to the reference (committed vectors and oracle);
-hip_barnes_hut (FP32 walk on FP32-rounded inputs) close to the FP64 result."""
+hip_barnes_hut (FP32 walk on FP32-rounded inputs): with per-particle reference
+lists close to the FP64 result and to the reference's interaction counts; the
+default group walk refines those lists."""
 import ctypes as C
 
 import numpy as np
@@ -46,8 +48,8 @@ def oracle_bh(oracle, p, theta, eps):
     return st, q.acc(), s, keys, perm, root, rows
 
 
-def gpu_bh(nbx, p, theta, eps, algorithm):
-    e = nbx.Engine(p.n, theta=theta)
+def gpu_bh(nbx, p, theta, eps, algorithm, reference_lists=False):
+    e = nbx.Engine(p.n, theta=theta, reference_lists=reference_lists)
     e.upload_view(p.view)
     e.compute_force(algorithm, eps=eps)
     return e, e.accelerations(), e.stats()
@@ -130,7 +132,7 @@ def test_bh_two_galaxy_fp32_inputs_keys_order_topology(nbx, oracle):
     st, oacc, os_, okeys, operm, oroot, orows = oracle_bh(oracle, p, 0.7, 0.01)
     assert st == 0
     raw = from_gen(d, fp32=False)
-    e, acc, s = gpu_bh(nbx, raw, 0.7, 0.01, "hip_barnes_hut")
+    e, acc, s = gpu_bh(nbx, raw, 0.7, 0.01, "hip_barnes_hut", reference_lists=True)
     root, keys, perm = e.bh_morton()
     rows = e.bh_export_nodes()
     e.close()
@@ -146,6 +148,37 @@ def test_bh_two_galaxy_fp32_inputs_keys_order_topology(nbx, oracle):
     print(f"fp32 walk rms rel err {err:.3e}; interactions gpu {s_t[3] + s_t[4]} cpu {o_t[3] + o_t[4]}")
     assert err < 1e-4
     assert abs((s_t[3] + s_t[4]) - (o_t[3] + o_t[4])) <= 1e-3 * (o_t[3] + o_t[4])
+
+
+@pytest.mark.parametrize("n", [200_000, 4_500_000])  # one and two queries per lane
+def test_group_walk_refines_reference_lists(nbx, oracle, n):
+    """The default FP32 group walk accepts a cell only when the whole group's
+    box passes the reference test, so it makes at least the reference's
+    interactions and is at least as accurate (FP64 direct sum on a sample)."""
+    d = nbx.two_galaxy(n, seed=42)
+    idx = np.sort(np.random.default_rng(3).choice(n, 2048, replace=False))
+    out = {}
+    for ref_lists in (True, False):
+        with nbx.Engine(n, theta=0.7, reference_lists=ref_lists) as e:
+            e.upload(d["pos"], d["m"])
+            e.compute_force("hip_barnes_hut", eps=0.01)
+            s = e.stats()
+            acc = e.accelerations()
+            ref = e.direct_fp64_targets(idx, 1.0, 0.01)
+        err = np.linalg.norm(acc[idx] - ref, axis=1) / np.linalg.norm(ref, axis=1)
+        out[ref_lists] = (s["accepted_approximations"] + s["exact_interactions"], err)
+        print(f"reference_lists={ref_lists}: interactions {out[ref_lists][0]} "
+              f"median err {np.median(err):.3e} p99 {np.quantile(err, 0.99):.3e}")
+    assert out[False][0] >= out[True][0]
+    assert np.median(out[False][1]) <= np.median(out[True][1])
+    assert np.quantile(out[False][1], 0.99) <= 1.05 * np.quantile(out[True][1], 0.99)
+    # the group walk visits the same tree: momentum of the force field stays ~0
+    with nbx.Engine(n, theta=0.7) as e:
+        e.upload(d["pos"], d["m"])
+        e.compute_force("hip_barnes_hut", eps=0.01)
+        a = e.accelerations()
+    f = (d["m"][:, None] * a).sum(0)
+    assert np.linalg.norm(f) < 1e-3 * (d["m"][:, None] * np.abs(a)).sum()
 
 
 def test_bh_fp64_two_galaxy_bit_identical(nbx, oracle):

@@ -188,7 +188,7 @@ def _walk_features_numpy(rows, perm, pos32, m32, root_h, theta, eps2):
 def test_feature_walk_matches_numpy_restatement(nbx):
     d = nbx.plummer(3000, seed=11)
     theta, eps = 1.0, 0.01
-    with nbx.Engine(3000, theta=theta) as e:
+    with nbx.Engine(3000, theta=theta, reference_lists=True) as e:
         e.upload(d["pos"], d["m"])
         feats, frames = e.bh_features(1.0, eps, frames=True)
         acc_feat = e.accelerations()
@@ -233,6 +233,34 @@ def test_feature_walk_matches_numpy_restatement(nbx):
     assert np.abs(frames[ok, 1].astype(np.float64) - e2[ok]).max() < 1e-3
 
 
+def test_group_walk_features_are_consistent(nbx):
+    """Default engine: the feature walk is the group walk. Its accelerations are
+    the plain group walk's bit for bit, the per-particle counts add up to the
+    walk's totals (every member of a group shares the list length), frames
+    are orthonormal and the tensor features are within their bounds."""
+    d = nbx.two_galaxy(50_000, seed=8)
+    with nbx.Engine(50_000, theta=1.0) as e:
+        e.upload(d["pos"], d["m"])
+        feats, frames = e.bh_features(1.0, 0.01, frames=True)
+        acc_feat = e.accelerations()
+        stats = e.stats()
+        e.compute_force("hip_barnes_hut", 1.0, 0.01)
+        assert np.array_equal(acc_feat, e.accelerations())
+        _, _, perm = e.bh_morton()
+    cells = np.rint(np.exp2(feats[:, 8].astype(np.float64)) - 1)
+    leaves = np.rint(np.exp2(feats[:, 9].astype(np.float64)) - 1)
+    assert cells.sum() == stats["accepted_approximations"]
+    assert leaves.sum() == stats["exact_interactions"]
+    g = np.einsum("nij,nkj->nik", frames.astype(np.float64), frames.astype(np.float64))
+    assert np.abs(g - np.eye(3)).max() < 1e-5
+    f = feats.astype(np.float64)
+    has = cells > 0
+    assert np.all(f[has, 1] >= 1.0 - 1e-6)        # accepted cells: d / edge > 1 / theta = 1
+    assert np.all((f[:, 4:8] >= -1e-4) & (f[:, 4:8] <= 1 + 1e-4))
+    assert np.all(f[:, 6] <= f[:, 7] + 1e-5)
+    assert not f[:, 12:].any()
+
+
 def test_correction_kernel_matches_host_formula(nbx, learned):
     d = nbx.plummer(20000, seed=12)
     p, mean, std = _random_model(7)
@@ -257,26 +285,33 @@ def test_correction_kernel_matches_host_formula(nbx, learned):
 
 
 def test_without_model_hip_bh_learned_is_plain_bh(nbx, learned):
+    """No model: hip_bh_learned is hip_barnes_hut, bit for bit (default group
+    walk and reference lists alike). A zero model leaves the corrected path's
+    base, the per-particle walk with reference lists, unchanged."""
     d = nbx.two_galaxy(30000, seed=3)
-    with nbx.Engine(30000, theta=1.0) as e:
-        e.upload(d["pos"], d["m"])
-        e.compute_force("hip_barnes_hut", 1.0, 0.01)
-        plain = e.accelerations()
-        e.compute_force("hip_bh_learned", 1.0, 0.01)
-        assert np.array_equal(e.accelerations(), plain)
-        with learned.Mlp() as m:   # a zero model changes nothing either
-            m.set_params(np.zeros(m.n_params), np.zeros(16), np.ones(16), 1.0)
-            e.set_mlp(m)
+    for ref_lists in (False, True):
+        with nbx.Engine(30000, theta=1.0, reference_lists=ref_lists) as e:
+            e.upload(d["pos"], d["m"])
+            e.compute_force("hip_barnes_hut", 1.0, 0.01)
+            plain = e.accelerations()
             e.compute_force("hip_bh_learned", 1.0, 0.01)
             assert np.array_equal(e.accelerations(), plain)
-            e.set_mlp(None)
-        e.compute_force("hip_bh_learned", 1.0, 0.01)
-        assert np.array_equal(e.accelerations(), plain)
+            if ref_lists:
+                with learned.Mlp() as m:
+                    m.set_params(np.zeros(m.n_params), np.zeros(16), np.ones(16), 1.0)
+                    e.set_mlp(m)
+                    e.compute_force("hip_bh_learned", 1.0, 0.01)
+                    assert np.array_equal(e.accelerations(), plain)
+                    e.set_mlp(None)
+            e.compute_force("hip_bh_learned", 1.0, 0.01)
+            assert np.array_equal(e.accelerations(), plain)
 
 
 def _errors(nbx, learned, d, mlp, eps=0.01):
+    """Plain walk with per-particle reference lists (the corrected path's base)
+    vs the corrected walk, both against the FP64 direct sum."""
     n = len(d["m"])
-    with nbx.Engine(n, theta=1.0) as e:
+    with nbx.Engine(n, theta=1.0, reference_lists=True) as e:
         e.upload(d["pos"], d["m"])
         e.compute_force("hip_direct_fp64", 1.0, eps)
         ref = e.accelerations()
