@@ -370,9 +370,7 @@ __global__ void tree_order_kernel(const uint32_t* __restrict__ asc_rank,
                                   const uint64_t* __restrict__ ext_asc,
                                   const double4* __restrict__ b4, int n, uint32_t* __restrict__ tperm, uint32_t* __restrict__ trank,
                                   uint64_t* __restrict__ tword, uint64_t* __restrict__ ext_tree,
-                                  double* __restrict__ sx, double* __restrict__ sy,
-                                  double* __restrict__ sz, double* __restrict__ sm,
-                                  float4* __restrict__ s4) {
+                                  double4* __restrict__ sp4, float4* __restrict__ s4) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) {
         return;
@@ -390,58 +388,158 @@ __global__ void tree_order_kernel(const uint32_t* __restrict__ asc_rank,
         }
     }
     const double4 p = b4[o];
-    const double x = p.x, y = p.y, z = p.z, m = p.w;
-    sx[t] = x;
-    sy[t] = y;
-    sz[t] = z;
-    sm[t] = m;
-    s4[t] = make_float4(static_cast<float>(x), static_cast<float>(y), static_cast<float>(z),
-                        static_cast<float>(m));
+    sp4[t] = p;
+    s4[t] = make_float4(static_cast<float>(p.x), static_cast<float>(p.y), static_cast<float>(p.z),
+                        static_cast<float>(p.w));
 }
 
 // ---------------------------------------------------------------- topology
-// delta[t] = common levels of tree positions t and t+1; cnt[t] = internal
-// nodes started at t (depths delta[t-1]+1 .. delta[t]); dcount[d] = internal
-// nodes at depth d (block-aggregated in LDS before one global add per depth).
+// Internal cells are numbered by level index L = doff[d] + k: the k-th cell of
+// depth d in tree order (doff = exclusive prefix of the per-depth counts), so
+// each depth is one contiguous range for the centre-of-mass pass. Tree
+// position t starts the cells of depths (delta[t-1], delta[t]] and ends those
+// of depths (delta[t], delta[t-1]]; cells of one depth are disjoint, so the
+// k-th start of depth d and the k-th end of depth d belong to the same cell.
+// The topology kernels give each workgroup one tile of kTopoTile consecutive
+// positions: per-tile counts of starts and ends per depth, scanned over the
+// tiles, plus wave ballots inside a tile, give every start and end its rank
+// in position order, with no search and no shared global counters.
+constexpr int kTopoItems = 8;
+constexpr int kTopoTile = kThreads * kTopoItems;
+constexpr int kTopoWaves = kThreads / 64;
+
+// Internal cells by level index.
+struct InternalSoA {
+    double4* com;     // FP64 centre of mass (x, y, z) and mass
+    int4* rec;        // node index, skip (node index past the subtree), first tree position, particles
+    int32_t* depth;
+};
+
+__device__ inline int wave_min_int(int v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const int o = __shfl_xor(v, off);
+        v = o < v ? o : v;
+    }
+    return v;
+}
+
+__device__ inline int wave_max_int(int v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const int o = __shfl_xor(v, off);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+// delta[t] = common levels of tree positions t and t+1 (-1 for the last);
+// cnt[t] = cells started at t; tile_hist[(k * kMaxLevels + d) * tiles + b] =
+// cells of depth d started (k = 0) or ended (k = 1) in tile b.
 __global__ __launch_bounds__(kThreads) void delta_kernel(const uint64_t* __restrict__ w,
                                                         const uint64_t* __restrict__ ext, int n,
-                                                        int32_t* __restrict__ delta,
+                                                        int tiles, int32_t* __restrict__ delta,
                                                         uint32_t* __restrict__ cnt,
                                                         unsigned* __restrict__ flags,
-                                                        unsigned* __restrict__ dcount) {
-    __shared__ unsigned hist[kMaxLevels];
-    for (int d = threadIdx.x; d < kMaxLevels; d += kThreads) {
+                                                        unsigned* __restrict__ tile_hist) {
+    __shared__ unsigned hist[2 * kMaxLevels];
+    __shared__ unsigned tile_flags;
+    for (int d = threadIdx.x; d < 2 * kMaxLevels; d += kThreads) {
         hist[d] = 0;
     }
+    if (threadIdx.x == 0) {
+        tile_flags = 0;
+    }
     __syncthreads();
-    const int t = blockIdx.x * kThreads + threadIdx.x;
-    if (t < n) {
+    unsigned mine = 0;
+    for (int i = 0; i < kTopoItems; ++i) {
+        const int t = blockIdx.x * kTopoTile + i * kThreads + threadIdx.x;
+        if (t >= n) {
+            break;
+        }
         const int dprev = t > 0 ? common_levels(w, ext, t - 1, t) : -1;
         const int dcur = t < n - 1 ? common_levels(w, ext, t, t + 1) : -1;
         delta[t] = dcur;
-        const int lo = dprev + 1;
-        const int c = dcur >= lo ? dcur - lo + 1 : 0;
+        const int c = dcur > dprev ? dcur - dprev : 0;
         cnt[t] = static_cast<uint32_t>(c);
         const int top = dcur < kMaxLevels ? dcur : kMaxLevels - 1;
-        for (int d = lo; d <= top; ++d) {
+        for (int d = dprev + 1; d <= top; ++d) {
             atomicAdd(&hist[d], 1u);
         }
+        const int etop = dprev < kMaxLevels ? dprev : kMaxLevels - 1;
+        for (int d = dcur + 1; d <= etop; ++d) {
+            atomicAdd(&hist[kMaxLevels + d], 1u);
+        }
         if (ext == nullptr && dcur >= 21) {
-            atomicOr(&flags[FLAG_TIE], 1u);
+            mine |= 1u;
         }
         if (dcur >= kMaxLevels) {
-            atomicOr(&flags[FLAG_OVERFLOW], 1u);
+            mine |= 2u;
         }
     }
+    if (mine != 0) {
+        atomicOr(&tile_flags, mine);
+    }
     __syncthreads();
-    for (int d = threadIdx.x; d < kMaxLevels; d += kThreads) {
-        if (hist[d] != 0) {
-            atomicAdd(&dcount[d], hist[d]);
+    for (int d = threadIdx.x; d < 2 * kMaxLevels; d += kThreads) {
+        tile_hist[static_cast<size_t>(d) * tiles + blockIdx.x] = hist[d];
+    }
+    if (threadIdx.x == 0) {
+        if (tile_flags & 1u) {
+            atomicOr(&flags[FLAG_TIE], 1u);
+        }
+        if (tile_flags & 2u) {
+            atomicOr(&flags[FLAG_OVERFLOW], 1u);
         }
     }
 }
 
-// Exclusive offsets of the per-depth internal-node lists.
+// One workgroup per (kind, depth) row: tile_base[row * tiles + b] = sum of
+// tile_hist[row * tiles + b'] over b' < b; dcount[d] = the depth's cell count.
+__global__ __launch_bounds__(kThreads) void depth_scan_kernel(const unsigned* __restrict__ tile_hist,
+                                                             int tiles,
+                                                             unsigned* __restrict__ tile_base,
+                                                             unsigned* __restrict__ dcount) {
+    __shared__ unsigned wave_sum[kTopoWaves];
+    const int row_id = blockIdx.x;
+    const unsigned* row = tile_hist + static_cast<size_t>(row_id) * tiles;
+    unsigned* out = tile_base + static_cast<size_t>(row_id) * tiles;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    unsigned running = 0;
+    for (int base = 0; base < tiles; base += kThreads) {
+        const int i = base + threadIdx.x;
+        const unsigned v = i < tiles ? row[i] : 0u;
+        unsigned x = v;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const unsigned y = __shfl_up(x, off, 64);
+            if (lane >= off) {
+                x += y;
+            }
+        }
+        if (lane == 63) {
+            wave_sum[wv] = x;
+        }
+        __syncthreads();
+        unsigned before = running, total = 0;
+#pragma unroll
+        for (int k = 0; k < kTopoWaves; ++k) {
+            const unsigned ws = wave_sum[k];
+            before += k < wv ? ws : 0u;
+            total += ws;
+        }
+        if (i < tiles) {
+            out[i] = before + x - v;
+        }
+        running += total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && row_id < kMaxLevels) {
+        dcount[row_id] = running;
+    }
+}
+
+// Exclusive offsets of the per-depth level-index ranges.
 __global__ void depth_offsets_kernel(const unsigned* __restrict__ dcount,
                                      unsigned* __restrict__ doff) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
@@ -453,179 +551,212 @@ __global__ void depth_offsets_kernel(const unsigned* __restrict__ dcount,
     }
 }
 
-struct NodeSoA {
-    double* cx;
-    double* cy;
-    double* cz;
-    double* m;
-    int32_t* skip;
-    int32_t* depth;
-    int32_t* start;
-    int32_t* count;
-    int32_t* minpos;
+// Per-tile rank bookkeeping shared by ends_kernel and fill_kernel: the tile is
+// walked in chunks of kThreads positions; per chunk each wave records how many
+// of its lanes hold each depth (wcount), and `running` carries the counts of
+// the tile's earlier chunks.
+struct TileRanks {
+    unsigned running[kMaxLevels];
+    unsigned wcount[kTopoWaves][kMaxLevels];
+    unsigned doff[kMaxLevels];
 };
 
-__device__ inline int lo_of(const int32_t* delta, int t) {
-    return t == 0 ? 0 : delta[t - 1] + 1;
+__device__ inline void tile_ranks_init(TileRanks& r, const unsigned* __restrict__ doff) {
+    for (int d = threadIdx.x; d < kMaxLevels; d += kThreads) {
+        r.running[d] = 0;
+        r.doff[d] = doff[d];
+    }
 }
 
-__device__ inline int internal_index(const uint32_t* scan, const int32_t* delta, int s, int d) {
-    return static_cast<int>(scan[s]) + s + (d - lo_of(delta, s));
+__device__ inline void tile_ranks_clear_chunk(TileRanks& r) {
+    for (int k = threadIdx.x; k < kTopoWaves * kMaxLevels; k += kThreads) {
+        (&r.wcount[0][0])[k] = 0;
+    }
 }
 
-__device__ inline int leaf_index(const uint32_t* scan, const uint32_t* cnt, int t) {
-    return static_cast<int>(scan[t] + cnt[t]) + t;
+// Lanes holding depth d (in [a, b]) counted per wave, for d in [wlo, whi].
+__device__ inline void tile_ranks_count(TileRanks& r, int wlo, int whi, int a, int b) {
+    const int wv = threadIdx.x >> 6;
+    for (int d = wlo; d <= whi; ++d) {
+        const unsigned long long m = __ballot(a <= d && d <= b);
+        if ((threadIdx.x & 63) == 0) {
+            r.wcount[wv][d] = static_cast<unsigned>(__popcll(m));
+        }
+    }
 }
 
-__device__ inline int internal_id(const uint32_t* scan, const int32_t* delta, int s, int d) {
-    return static_cast<int>(scan[s]) + (d - lo_of(delta, s));
+__device__ inline void tile_ranks_advance(TileRanks& r) {
+    for (int d = threadIdx.x; d < kMaxLevels; d += kThreads) {
+        unsigned s = 0;
+#pragma unroll
+        for (int k = 0; k < kTopoWaves; ++k) {
+            s += r.wcount[k][d];
+        }
+        r.running[d] += s;
+    }
 }
 
-// Internal id of the depth-d cell that contains tree position t (d >= 0).
-__device__ inline int cell_id(const uint64_t* w, const uint64_t* ext, const uint32_t* scan,
-                              const int32_t* delta, int t, int d) {
-    const int s = d >= lo_of(delta, t) ? t : run_start(w, ext, t, d);
-    return internal_id(scan, delta, s, d);
+// Rank, among the holders of depth d, of the lanes before this one: earlier
+// tiles, earlier chunks, earlier waves of this chunk, earlier lanes.
+__device__ inline unsigned tile_rank_before(const TileRanks& r, const unsigned* __restrict__ base_row,
+                                            int tiles, int d, unsigned long long mask) {
+    const int wv = threadIdx.x >> 6;
+    unsigned before = base_row[static_cast<size_t>(d) * tiles + blockIdx.x] + r.running[d];
+    for (int k = 0; k < wv; ++k) {
+        before += r.wcount[k][d];
+    }
+    const unsigned lane = threadIdx.x & 63;
+    return before + static_cast<unsigned>(__popcll(mask & ((1ull << lane) - 1ull)));
 }
 
-// Static fields of the internal nodes, the walk records of the leaves, the
-// per-depth lists of internal nodes (by internal id), and every node's slot
-// in its parent's child table kids[parent_internal_id * 8 + octant]
-// (internal child: its node index; leaf child: -(tree position + 2); the
-// per-node SoA arrays hold internal nodes only).
+// endpos[L] = last tree position of cell L.
+__global__ __launch_bounds__(kThreads) void ends_kernel(const int32_t* __restrict__ delta, int n,
+                                                       int tiles,
+                                                       const unsigned* __restrict__ doff,
+                                                       const unsigned* __restrict__ end_base,
+                                                       int* __restrict__ endpos) {
+    __shared__ TileRanks r;
+    tile_ranks_init(r, doff);
+    for (int i = 0; i < kTopoItems; ++i) {
+        tile_ranks_clear_chunk(r);
+        __syncthreads();
+        const int e = blockIdx.x * kTopoTile + i * kThreads + threadIdx.x;
+        const bool valid = e < n && e > 0;
+        const int a = valid ? delta[e] + 1 : 0;
+        const int b = valid ? delta[e - 1] : -1;
+        const bool has = a <= b;
+        const int wlo = wave_min_int(has ? a : 0x7FFFFFFF);
+        const int whi = wave_max_int(has ? b : -1);
+        tile_ranks_count(r, wlo, whi, has ? a : 1, has ? b : 0);
+        __syncthreads();
+        for (int d = wlo; d <= whi; ++d) {
+            const bool in = has && a <= d && d <= b;
+            const unsigned long long m = __ballot(in);
+            if (in) {
+                endpos[r.doff[d] + tile_rank_before(r, end_base, tiles, d, m)] = e;
+            }
+        }
+        __syncthreads();
+        tile_ranks_advance(r);
+        __syncthreads();
+    }
+}
+
+// Static fields of the internal cells (by level index), the walk records of
+// the leaves, and every node's slot in its parent's child table kids[parent L
+// * 8 + octant] (internal child: its level index; leaf child: -(tree position
+// + 2)). A position's cells are visited shallow to deep, so each one's parent
+// is the cell visited just before it: the depth-delta[t-1] cell that holds
+// t-1 and t (already started) for the shallowest, the previous one otherwise.
 __global__ __launch_bounds__(kThreads) void fill_kernel(
     const uint64_t* __restrict__ w, const uint64_t* __restrict__ ext,
     const int32_t* __restrict__ delta, const uint32_t* __restrict__ cnt,
-    const uint32_t* __restrict__ scan, int n, const double* __restrict__ sx,
-    const double* __restrict__ sy, const double* __restrict__ sz, const double* __restrict__ sm,
-    double root_h, NodeSoA nd, NodeF64* __restrict__ n64, NodeF32* __restrict__ n32,
-    const unsigned* __restrict__ doff, unsigned* __restrict__ dcursor, int* __restrict__ dlist,
-    int* __restrict__ inode, int* __restrict__ kids) {
-    __shared__ unsigned local[kMaxLevels];
-    __shared__ unsigned base[kMaxLevels];
-    for (int d = threadIdx.x; d < kMaxLevels; d += kThreads) {
-        local[d] = 0;
-    }
-    __syncthreads();
-    const int t = blockIdx.x * kThreads + threadIdx.x;
-    const bool valid = t < n;
-    const int lo = valid ? lo_of(delta, t) : 0;
-    const int hi = valid ? delta[t] : -1;
-    for (int d = lo; d <= hi; ++d) {
-        atomicAdd(&local[d], 1u);
-    }
-    __syncthreads();
-    for (int d = threadIdx.x; d < kMaxLevels; d += kThreads) {
-        base[d] = local[d] != 0 ? atomicAdd(&dcursor[d], local[d]) : 0u;
-        local[d] = 0;
-    }
-    __syncthreads();
-    if (!valid) {
-        return;
-    }
-    for (int d = lo; d <= hi; ++d) {
-        const int idx = internal_index(scan, delta, t, d);
-        const int iid = internal_id(scan, delta, t, d);
-        const int e = run_end(w, ext, n, t, d);
-        nd.skip[idx] = leaf_index(scan, cnt, e) + 1;
-        nd.depth[idx] = d;
-        nd.start[idx] = t;
-        nd.count[idx] = e - t + 1;
-        inode[iid] = idx;
-        const unsigned slot = base[d] + atomicAdd(&local[d], 1u);
-        dlist[doff[d] + slot] = iid;
-        if (d > 0) {
-            const int parent = d - 1 >= lo ? iid - 1 : cell_id(w, ext, scan, delta, t, d - 1);
-            kids[parent * 8 + digit_at(w, ext, t, d - 1)] = idx;
+    const uint32_t* __restrict__ scan, int n, int tiles, const double4* __restrict__ sp4,
+    const double* __restrict__ root, InternalSoA in, NodeF64* __restrict__ n64,
+    NodeF32* __restrict__ n32, const unsigned* __restrict__ doff,
+    const unsigned* __restrict__ start_base, const int* __restrict__ endpos,
+    int* __restrict__ kids) {
+    __shared__ TileRanks r;
+    tile_ranks_init(r, doff);
+    const double root_h = root[3];
+    for (int i = 0; i < kTopoItems; ++i) {
+        tile_ranks_clear_chunk(r);
+        __syncthreads();
+        const int t = blockIdx.x * kTopoTile + i * kThreads + threadIdx.x;
+        const bool valid = t < n;
+        const int dprev = valid && t > 0 ? delta[t - 1] : -1;
+        const int hi = valid ? delta[t] : -1;
+        const int lo = dprev + 1;
+        const bool has = valid && lo <= hi;
+        const int wlo = wave_min_int(valid ? (dprev > 0 ? dprev : 0) : 0x7FFFFFFF);
+        const int whi = wave_max_int(valid ? (dprev > hi ? dprev : hi) : -1);
+        tile_ranks_count(r, wlo, whi, has ? lo : 1, has ? hi : 0);
+        __syncthreads();
+        int parent = -1;
+        const uint32_t sc = valid ? scan[t] : 0u;
+        for (int d = wlo; d <= whi; ++d) {
+            const bool starts = has && lo <= d && d <= hi;
+            const unsigned long long m = __ballot(starts);
+            if (valid && (starts || d == dprev)) {
+                // a start of its own, or the latest depth-dprev start before t
+                const int L = static_cast<int>(r.doff[d] + tile_rank_before(r, start_base, tiles, d, m)) -
+                              (starts ? 0 : 1);
+                if (starts) {
+                    const int e = endpos[L];
+                    const int idx = static_cast<int>(sc) + t + (d - lo);
+                    const int skip = static_cast<int>(scan[e] + cnt[e]) + e + 1;
+                    in.rec[L] = make_int4(idx, skip, t, e - t + 1);
+                    in.depth[L] = d;
+                    if (parent >= 0) {
+                        kids[parent * 8 + digit_at(w, ext, t, d - 1)] = L;
+                    }
+                }
+                parent = L;
+            }
         }
+        if (valid) {
+            const int depth = 1 + (dprev > hi ? dprev : hi);
+            const int li = static_cast<int>(sc + cnt[t]) + t;
+            if (depth > 0 && parent >= 0) {
+                kids[parent * 8 + digit_at(w, ext, t, depth - 1)] = -(t + 2);
+            }
+            const double4 q = sp4[t];
+            if (n64 != nullptr) {   // FP64 records only for the exact FP64 walk
+                NodeF64 a;
+                a.cx = q.x;
+                a.cy = q.y;
+                a.cz = q.z;
+                a.m = q.w;
+                a.h = ldexp(root_h, -depth);
+                a.skip = li + 1;
+                a.leaf = t;
+                n64[li] = a;
+            }
+            NodeF32 b;
+            b.cx = static_cast<float>(q.x);
+            b.cy = static_cast<float>(q.y);
+            b.cz = static_cast<float>(q.z);
+            b.m = static_cast<float>(q.w);
+            b.size2 = 0.f;
+            b.skip = li + 1;
+            b.leaf = t;
+            b.inv_size2 = 0.f;
+            n32[li] = b;
+        }
+        __syncthreads();
+        tile_ranks_advance(r);
+        __syncthreads();
     }
-    const int dprev = t > 0 ? delta[t - 1] : -1;
-    const int depth = 1 + (dprev > hi ? dprev : hi);
-    const int li = leaf_index(scan, cnt, t);
-    if (depth > 0) {
-        // leaves are referenced by tree position, tagged -(t + 2): the
-        // centre-of-mass pass reads them straight from the tree-order arrays
-        const int parent = depth - 1 == hi && hi >= lo ? internal_id(scan, delta, t, hi)
-                                                       : cell_id(w, ext, scan, delta, t, depth - 1);
-        kids[parent * 8 + digit_at(w, ext, t, depth - 1)] = -(t + 2);
-    }
-    const double x = sx[t], y = sy[t], z = sz[t], m = sm[t];
-    if (n64 != nullptr) {   // FP64 records only for the exact FP64 walk
-        NodeF64 a;
-        a.cx = x;
-        a.cy = y;
-        a.cz = z;
-        a.m = m;
-        a.h = ldexp(root_h, -depth);
-        a.skip = li + 1;
-        a.leaf = t;
-        n64[li] = a;
-    }
-    NodeF32 b;
-    b.cx = static_cast<float>(x);
-    b.cy = static_cast<float>(y);
-    b.cz = static_cast<float>(z);
-    b.m = static_cast<float>(m);
-    b.size2 = 0.f;
-    b.skip = li + 1;
-    b.leaf = t;
-    b.inv_size2 = 0.f;
-    n32[li] = b;
 }
 
 // One node of the centre-of-mass pass (barnes_hut.c:214-253): children
 // summed in octant order 0..7; a zero-mass cell keeps the position of the
 // particle that created it (lowest Morton rank in the cell; for the root the
 // caller's particle 0, barnes_hut.c:504-512). Children are one level deeper
-// and were finished by the previous launch.
-__device__ inline void com_node(int iid, const int* __restrict__ inode,
-                                const int* __restrict__ kids, NodeSoA nd,
-                                const uint32_t* __restrict__ trank,
-                                const double* __restrict__ sx, const double* __restrict__ sy,
-                                const double* __restrict__ sz, const double* __restrict__ sm,
-                                const double4* __restrict__ b4, double root_h,
-                                NodeF64* __restrict__ n64, NodeF32* __restrict__ n32) {
-    const int p = inode[iid];
-    const int end = nd.skip[p];
-    int kid[8];
-#pragma unroll
-    for (int o = 0; o < 8; ++o) {
-        kid[o] = kids[iid * 8 + o];
-    }
+// and were finished by the previous launch (or barrier). A child is one
+// 32-byte record: the tree-order particle or the child's centre of mass.
+__device__ inline void com_node(int L, int depth, const int* __restrict__ kids,
+                                InternalSoA in, const uint32_t* __restrict__ trank,
+                                const double4* __restrict__ sp4, const double4* __restrict__ b4,
+                                double root_h, NodeF64* __restrict__ n64,
+                                NodeF32* __restrict__ n32) {
+    const int4 r = in.rec[L];
+    const int p = r.x;
+    const int4 k0 = reinterpret_cast<const int4*>(kids)[2 * L];
+    const int4 k1 = reinterpret_cast<const int4*>(kids)[2 * L + 1];
+    const int kid[8] = {k0.x, k0.y, k0.z, k0.w, k1.x, k1.y, k1.z, k1.w};
     double mass = 0.0, x = 0.0, y = 0.0, z = 0.0;
-    int minpos = -1;
-    uint32_t minrank = 0xFFFFFFFFu;
 #pragma unroll
     for (int o = 0; o < 8; ++o) {
         const int c = kid[o];
         if (c == -1) {
             continue;
         }
-        double cm, cx, cy, cz;
-        int mp;
-        if (c < -1) {   // leaf at tree position -c - 2
-            const int t = -c - 2;
-            cm = sm[t];
-            cx = sx[t];
-            cy = sy[t];
-            cz = sz[t];
-            mp = t;
-        } else {
-            cm = nd.m[c];
-            cx = nd.cx[c];
-            cy = nd.cy[c];
-            cz = nd.cz[c];
-            mp = nd.minpos[c];
-        }
-        mass += cm;
-        x += cm * cx;
-        y += cm * cy;
-        z += cm * cz;
-        const uint32_t r = trank[mp];
-        if (r < minrank) {
-            minrank = r;
-            minpos = mp;
-        }
+        const double4 q = c < -1 ? sp4[-c - 2] : in.com[c];
+        mass += q.w;
+        x += q.w * q.x;
+        y += q.w * q.y;
+        z += q.w * q.z;
     }
     double ox, oy, oz;
     if (mass != 0.0) {
@@ -638,16 +769,22 @@ __device__ inline void com_node(int iid, const int* __restrict__ inode,
         oy = p0.y;
         oz = p0.z;
     } else {
-        ox = sx[minpos];
-        oy = sy[minpos];
-        oz = sz[minpos];
+        // the cell's particles are tree positions r.z .. r.z + r.w - 1
+        int minpos = r.z;
+        uint32_t minrank = trank[r.z];
+        for (int t = r.z + 1; t < r.z + r.w; ++t) {
+            const uint32_t k = trank[t];
+            if (k < minrank) {
+                minrank = k;
+                minpos = t;
+            }
+        }
+        const double4 q = sp4[minpos];
+        ox = q.x;
+        oy = q.y;
+        oz = q.z;
     }
-    nd.m[p] = mass;
-    nd.cx[p] = ox;
-    nd.cy[p] = oy;
-    nd.cz[p] = oz;
-    nd.minpos[p] = minpos;
-    const int depth = nd.depth[p];
+    in.com[L] = make_double4(ox, oy, oz, mass);
     const double h = ldexp(root_h, -depth);
     if (n64 != nullptr) {
         NodeF64 a;
@@ -656,7 +793,7 @@ __device__ inline void com_node(int iid, const int* __restrict__ inode,
         a.cz = oz;
         a.m = mass;
         a.h = h;
-        a.skip = end;
+        a.skip = r.y;
         a.leaf = -1;
         n64[p] = a;
     }
@@ -666,40 +803,38 @@ __device__ inline void com_node(int iid, const int* __restrict__ inode,
     b.cz = static_cast<float>(oz);
     b.m = static_cast<float>(mass);
     b.size2 = static_cast<float>(4.0 * h * h);
-    b.skip = end;
+    b.skip = r.y;
     b.leaf = -1;
     b.inv_size2 = 1.f / b.size2;
     n32[p] = b;
 }
 
-// One depth level per launch (large levels).
+// One depth level per launch (large levels): cells first .. first + count - 1.
 __global__ __launch_bounds__(kThreads) void com_level_kernel(
-    const int* __restrict__ list, int count, const int* __restrict__ inode,
-    const int* __restrict__ kids, NodeSoA nd, const uint32_t* __restrict__ trank,
-    const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
-    const double* __restrict__ sm, const double4* __restrict__ b4, double root_h,
-    NodeF64* __restrict__ n64, NodeF32* __restrict__ n32) {
+    int first, int count, int depth, const int* __restrict__ kids,
+    InternalSoA in, const uint32_t* __restrict__ trank, const double4* __restrict__ sp4,
+    const double4* __restrict__ b4, const double* __restrict__ root, NodeF64* __restrict__ n64,
+    NodeF32* __restrict__ n32) {
     const int k = blockIdx.x * kThreads + threadIdx.x;
     if (k < count) {
-        com_node(list[k], inode, kids, nd, trank, sx, sy, sz, sm, b4, root_h, n64, n32);
+        com_node(first + k, depth, kids, in, trank, sp4, b4, root[3], n64, n32);
     }
 }
 
 // Consecutive small depth levels (deepest first) in one workgroup; the
 // barrier orders each level's results before the next level reads them.
 __global__ __launch_bounds__(1024) void com_small_levels_kernel(
-    const int* __restrict__ list, const unsigned* __restrict__ doff,
-    const unsigned* __restrict__ dcount, int d_hi, int d_lo, const int* __restrict__ inode,
-    const int* __restrict__ kids, NodeSoA nd,
-    const uint32_t* __restrict__ trank, const double* __restrict__ sx,
-    const double* __restrict__ sy, const double* __restrict__ sz, const double* __restrict__ sm,
-    const double4* __restrict__ b4, double root_h, NodeF64* __restrict__ n64,
+    const unsigned* __restrict__ doff,
+    const unsigned* __restrict__ dcount, int d_hi, int d_lo, const int* __restrict__ kids,
+    InternalSoA in, const uint32_t* __restrict__ trank, const double4* __restrict__ sp4,
+    const double4* __restrict__ b4, const double* __restrict__ root, NodeF64* __restrict__ n64,
     NodeF32* __restrict__ n32) {
+    const double root_h = root[3];
     for (int d = d_hi; d >= d_lo; --d) {
         const int count = static_cast<int>(dcount[d]);
-        const int* lst = list + doff[d];
+        const int first = static_cast<int>(doff[d]);
         for (int k = threadIdx.x; k < count; k += blockDim.x) {
-            com_node(lst[k], inode, kids, nd, trank, sx, sy, sz, sm, b4, root_h, n64, n32);
+            com_node(first + k, d, kids, in, trank, sp4, b4, root_h, n64, n32);
         }
         __syncthreads();
     }
@@ -709,8 +844,7 @@ __global__ __launch_bounds__(1024) void com_small_levels_kernel(
 // ---------------------------------------------------------------- walk
 // FP64 variant: the reference arithmetic (barnes_hut.c:574-613), bit-identical.
 __global__ __launch_bounds__(kThreads) void walk_f64_kernel(
-    const NodeF64* __restrict__ n64, int nodes, const double* __restrict__ sx,
-    const double* __restrict__ sy, const double* __restrict__ sz,
+    const NodeF64* __restrict__ n64, int nodes, const double4* __restrict__ sp4,
     const uint32_t* __restrict__ tperm, int n, double G, double eps2, double theta2,
     double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az,
     unsigned long long* __restrict__ counters) {
@@ -724,7 +858,8 @@ __global__ __launch_bounds__(kThreads) void walk_f64_kernel(
     const int tc = valid ? t : n - 1;
     int skip_until = valid ? 0 : 0x7FFFFFFF;
     unsigned approx = 0, exact = 0;
-    const double qx = sx[tc], qy = sy[tc], qz = sz[tc];
+    const double4 qp = sp4[tc];
+    const double qx = qp.x, qy = qp.y, qz = qp.z;
     double acx = 0.0, acy = 0.0, acz = 0.0;
     int i = 0;
     while (i < nodes) {
@@ -1225,26 +1360,19 @@ __global__ void gather_rank_word0_kernel(const uint64_t* __restrict__ tkey,
     }
 }
 
-NodeSoA node_soa(BHState& s) {
-    NodeSoA nd;
-    nd.cx = B<double>(s.ncx);
-    nd.cy = B<double>(s.ncy);
-    nd.cz = B<double>(s.ncz);
-    nd.m = B<double>(s.nm);
-    nd.skip = B<int32_t>(s.nskip);
-    nd.depth = B<int32_t>(s.ndepth);
-    nd.start = B<int32_t>(s.nstart);
-    nd.count = B<int32_t>(s.ncount);
-    nd.minpos = B<int32_t>(s.nminpos);
-    return nd;
+InternalSoA internal_soa(BHState& s) {
+    InternalSoA in;
+    in.com = B<double4>(s.icom);
+    in.rec = B<int4>(s.irec);
+    in.depth = B<int32_t>(s.idepth);
+    return in;
 }
 
 // Device word layout of s.flags: [0..7] flags, [8..8+129) depth counts,
-// [256..) depth offsets, [512..) depth cursors.
+// [256..256+129) depth offsets.
 constexpr int kDCount = 8;
 constexpr int kDOff = 256;
-constexpr int kDCursor = 512;
-constexpr int kFlagWords = 768;
+constexpr int kFlagWords = 512;
 
 }  // namespace
 
@@ -1288,12 +1416,13 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error)
     NBX_TRY(s.delta.need<int32_t>(n, error));
     NBX_TRY(s.cnt.need<uint32_t>(n, error));
     NBX_TRY(s.scan.need<uint32_t>(n + 1, error));
-    NBX_TRY(s.sx.need<double>(n, error));
-    NBX_TRY(s.sy.need<double>(n, error));
-    NBX_TRY(s.sz.need<double>(n, error));
-    NBX_TRY(s.sm.need<double>(n, error));
+    NBX_TRY(s.sp4.need<double4>(n, error));
     NBX_TRY(s.s4.need<float4>(n, error));
     NBX_TRY(s.counters.need<unsigned long long>(2, error));
+    const size_t tiles = (n + kTopoTile - 1) / kTopoTile;
+    NBX_TRY(s.tile_hist.need<unsigned>(2 * kMaxLevels * tiles, error));
+    NBX_TRY(s.tile_base.need<unsigned>(2 * kMaxLevels * tiles, error));
+    const int nt = static_cast<int>(tiles);
 
     double* root = B<double>(s.partial) + 6 * kBBoxBlocks;
     unsigned* flags = B<unsigned>(s.flags);
@@ -1311,34 +1440,48 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error)
     NBX_TRY(radix_sort_pairs(st, B<uint64_t>(s.sk0), B<uint32_t>(s.sv0), B<uint64_t>(s.sk1),
                              B<uint32_t>(s.sv1), n, 63, s.hist, error,
                              B<uint64_t>(s.mkey_sorted), B<uint32_t>(s.mperm)));
+
+    // Tree order, neighbour depths and per-depth node counts for a given
+    // ascending true-path order (null: Morton order), then one host read of
+    // the flags, counts and root cube.
     unsigned hflags[kDOff];
-    NBX_HIP(error, hipMemcpyAsync(hflags, flags, 8 * sizeof(unsigned), hipMemcpyDeviceToHost, st));
-    NBX_HIP(error, hipMemcpyAsync(s.root, root, 4 * sizeof(double), hipMemcpyDeviceToHost, st));
-    NBX_HIP(error, hipStreamSynchronize(st));
+    auto topology = [&](const uint32_t* asc_rank, const uint64_t* ext_asc, uint64_t* ext_t)
+        -> NBodySimProStatus {
+        tree_order_kernel<<<g, kThreads, 0, st>>>(
+            asc_rank, B<uint32_t>(s.mperm), B<uint64_t>(s.tkey), ext_asc, b4, ni,
+            B<uint32_t>(s.tperm), B<uint32_t>(s.trank), B<uint64_t>(s.tword), ext_t,
+            B<double4>(s.sp4), B<float4>(s.s4));
+        delta_kernel<<<nt, kThreads, 0, st>>>(B<uint64_t>(s.tword), ext_t, ni, nt,
+                                              B<int32_t>(s.delta), B<uint32_t>(s.cnt), flags,
+                                              B<unsigned>(s.tile_hist));
+        depth_scan_kernel<<<2 * kMaxLevels, kThreads, 0, st>>>(B<unsigned>(s.tile_hist), nt,
+                                                           B<unsigned>(s.tile_base),
+                                                           flags + kDCount);
+        NBX_HIP(error, hipGetLastError());
+        NBX_HIP(error, hipMemcpyAsync(hflags, flags, kDOff * sizeof(unsigned),
+                                      hipMemcpyDeviceToHost, st));
+        NBX_HIP(error, hipMemcpyAsync(s.root, root, 4 * sizeof(double), hipMemcpyDeviceToHost,
+                                      st));
+        NBX_HIP(error, hipStreamSynchronize(st));
+        return N_BODY_SIM_PRO_STATUS_OK;
+    };
+    // The Morton order is the tree order unless some true-path key differs
+    // from its Morton key (keys_kernel raises the flag): build on it and redo
+    // the topology in that rare case.
+    NBX_TRY(topology(nullptr, nullptr, nullptr));
     s.mismatch_path = hflags[FLAG_MISMATCH] != 0;
     s.tie_path = false;
-
-    // asc_rank[k]: Morton rank of the k-th particle in ascending true-path order.
-    const uint32_t* asc_rank = nullptr;
     if (s.mismatch_path) {
+        // asc_rank[k]: Morton rank of the k-th particle in ascending true-path order
         gather_u64_kernel<<<g, kThreads, 0, st>>>(B<uint64_t>(s.tkey), B<uint32_t>(s.mperm),
                                                   B<uint64_t>(s.sk0), B<uint32_t>(s.sv0), ni, 1, 0);
         NBX_TRY(radix_sort_pairs(st, B<uint64_t>(s.sk0), B<uint32_t>(s.sv0), B<uint64_t>(s.sk1),
                                  B<uint32_t>(s.sv1), n, 63, s.hist, error));
         NBX_HIP(error, hipMemcpyAsync(s.tmp32.p, s.sv0.p, n * sizeof(uint32_t),
                                       hipMemcpyDeviceToDevice, st));
-        asc_rank = B<uint32_t>(s.tmp32);
+        NBX_HIP(error, hipMemsetAsync(flags, 0, kFlagWords * sizeof(unsigned), st));
+        NBX_TRY(topology(B<uint32_t>(s.tmp32), nullptr, nullptr));
     }
-    tree_order_kernel<<<g, kThreads, 0, st>>>(
-        asc_rank, B<uint32_t>(s.mperm), B<uint64_t>(s.tkey), nullptr, b4, ni,
-        B<uint32_t>(s.tperm), B<uint32_t>(s.trank), B<uint64_t>(s.tword), nullptr, B<double>(s.sx),
-        B<double>(s.sy), B<double>(s.sz), B<double>(s.sm), B<float4>(s.s4));
-    delta_kernel<<<g, kThreads, 0, st>>>(B<uint64_t>(s.tword), nullptr, ni, B<int32_t>(s.delta),
-                                         B<uint32_t>(s.cnt), flags, flags + kDCount);
-    NBX_HIP(error, hipGetLastError());
-    NBX_HIP(error, hipMemcpyAsync(hflags, flags, kDOff * sizeof(unsigned), hipMemcpyDeviceToHost,
-                                  st));
-    NBX_HIP(error, hipStreamSynchronize(st));
 
     const uint64_t* ext_tree = nullptr;
     if (hflags[FLAG_TIE] != 0) {
@@ -1375,16 +1518,7 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error)
                                             hipMemcpyDeviceToDevice, st));
         }
         NBX_HIP(error, hipMemsetAsync(flags, 0, kFlagWords * sizeof(unsigned), st));
-        tree_order_kernel<<<g, kThreads, 0, st>>>(
-            P, B<uint32_t>(s.mperm), B<uint64_t>(s.tkey), ext_a, b4, ni,
-            B<uint32_t>(s.tperm), B<uint32_t>(s.trank), B<uint64_t>(s.tword), ext_t,
-            B<double>(s.sx), B<double>(s.sy), B<double>(s.sz), B<double>(s.sm), B<float4>(s.s4));
-        delta_kernel<<<g, kThreads, 0, st>>>(B<uint64_t>(s.tword), ext_t, ni, B<int32_t>(s.delta),
-                                             B<uint32_t>(s.cnt), flags, flags + kDCount);
-        NBX_HIP(error, hipGetLastError());
-        NBX_HIP(error, hipMemcpyAsync(hflags, flags, kDOff * sizeof(unsigned),
-                                      hipMemcpyDeviceToHost, st));
-        NBX_HIP(error, hipStreamSynchronize(st));
+        NBX_TRY(topology(P, ext_a, ext_t));
         ext_tree = ext_t;
     }
     if (hflags[FLAG_OVERFLOW] != 0) {
@@ -1401,17 +1535,10 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error)
     NBX_HIP(error, hipEventRecord(e->ev[2], st));
 
     const size_t nodes = n + internal;
-    NBX_TRY(s.ncx.need<double>(nodes, error));
-    NBX_TRY(s.ncy.need<double>(nodes, error));
-    NBX_TRY(s.ncz.need<double>(nodes, error));
-    NBX_TRY(s.nm.need<double>(nodes, error));
-    NBX_TRY(s.nskip.need<int32_t>(nodes, error));
-    NBX_TRY(s.ndepth.need<int32_t>(nodes, error));
-    NBX_TRY(s.nstart.need<int32_t>(nodes, error));
-    NBX_TRY(s.ncount.need<int32_t>(nodes, error));
-    NBX_TRY(s.nminpos.need<int32_t>(nodes, error));
-    NBX_TRY(s.nlist.need<int>(internal + 1, error));
-    NBX_TRY(s.ninode.need<int>(internal + 1, error));
+    NBX_TRY(s.icom.need<double4>(internal + 1, error));
+    NBX_TRY(s.irec.need<int4>(internal + 1, error));
+    NBX_TRY(s.idepth.need<int32_t>(internal + 1, error));
+    NBX_TRY(s.endpos.need<int>(internal + 1, error));
     NBX_TRY(s.nkids.need<int>(8 * internal + 8, error));
     NBX_HIP(error, hipMemsetAsync(s.nkids.p, 0xFF, (8 * internal + 8) * sizeof(int), st));
     // FP64 node records feed only the exact FP64 walk
@@ -1423,15 +1550,18 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error)
         s.has_n64 = true;
     }
     NBX_TRY(s.n32.need<NodeF32>(nodes, error));
-    NodeSoA nd = node_soa(s);
+    const InternalSoA in = internal_soa(s);
+    const double4* sp4 = B<double4>(s.sp4);
     depth_offsets_kernel<<<1, 1, 0, st>>>(flags + kDCount, flags + kDOff);
     s.ext_tree = ext_tree;
-    fill_kernel<<<g, kThreads, 0, st>>>(B<uint64_t>(s.tword), ext_tree, B<int32_t>(s.delta),
-                                        B<uint32_t>(s.cnt), B<uint32_t>(s.scan), ni,
-                                        B<double>(s.sx), B<double>(s.sy), B<double>(s.sz),
-                                        B<double>(s.sm), s.root[3], nd, n64_out,
-                                        B<NodeF32>(s.n32), flags + kDOff, flags + kDCursor,
-                                        B<int>(s.nlist), B<int>(s.ninode), B<int>(s.nkids));
+    const unsigned* start_base = B<unsigned>(s.tile_base);
+    const unsigned* end_base = start_base + static_cast<size_t>(kMaxLevels) * tiles;
+    ends_kernel<<<nt, kThreads, 0, st>>>(B<int32_t>(s.delta), ni, nt, flags + kDOff, end_base,
+                                         B<int>(s.endpos));
+    fill_kernel<<<nt, kThreads, 0, st>>>(B<uint64_t>(s.tword), ext_tree, B<int32_t>(s.delta),
+                                         B<uint32_t>(s.cnt), B<uint32_t>(s.scan), ni, nt, sp4,
+                                         root, in, n64_out, B<NodeF32>(s.n32), flags + kDOff,
+                                         start_base, B<int>(s.endpos), B<int>(s.nkids));
     NBX_HIP(error, hipGetLastError());
     NBX_HIP(error, hipEventRecord(e->ev[3], st));
     std::vector<unsigned> offs(kMaxLevels, 0);
@@ -1446,10 +1576,8 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error)
         const unsigned count = hflags[kDCount + d];
         if (count > kSmallLevel) {
             com_level_kernel<<<grid_for(count, kThreads), kThreads, 0, st>>>(
-                B<int>(s.nlist) + offs[d], static_cast<int>(count), B<int>(s.ninode),
-                B<int>(s.nkids), nd, B<uint32_t>(s.trank),
-                B<double>(s.sx), B<double>(s.sy), B<double>(s.sz), B<double>(s.sm), b4, s.root[3],
-                n64_out, B<NodeF32>(s.n32));
+                static_cast<int>(offs[d]), static_cast<int>(count), d, B<int>(s.nkids), in,
+                B<uint32_t>(s.trank), sp4, b4, root, n64_out, B<NodeF32>(s.n32));
             --d;
             continue;
         }
@@ -1458,10 +1586,8 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error)
             --lo;
         }
         com_small_levels_kernel<<<1, 1024, 0, st>>>(
-            B<int>(s.nlist), flags + kDOff, flags + kDCount, d, lo, B<int>(s.ninode),
-            B<int>(s.nkids), nd, B<uint32_t>(s.trank),
-            B<double>(s.sx), B<double>(s.sy), B<double>(s.sz), B<double>(s.sm), b4, s.root[3],
-            n64_out, B<NodeF32>(s.n32));
+            flags + kDOff, flags + kDCount, d, lo, B<int>(s.nkids), in,
+            B<uint32_t>(s.trank), sp4, b4, root, n64_out, B<NodeF32>(s.n32));
         d = lo - 1;
     }
     NBX_HIP(error, hipGetLastError());
@@ -1496,8 +1622,7 @@ NBodySimProStatus bh_walk(Engine* e, const NBodySimProGravity* g, bool fp64_exac
     }
     if (fp64_exact) {
         walk_f64_kernel<<<blocks, kThreads, 0, st>>>(
-            B<NodeF64>(s.n64), static_cast<int>(s.nodes), B<double>(s.sx), B<double>(s.sy),
-            B<double>(s.sz), B<uint32_t>(s.tperm), ni, g->gravitational_constant,
+            B<NodeF64>(s.n64), static_cast<int>(s.nodes), B<double4>(s.sp4), B<uint32_t>(s.tperm), ni, g->gravitational_constant,
             g->softening_squared, theta2, e->ax, e->ay, e->az, B<unsigned long long>(s.counters));
     } else if (!(e->flags & NBX_FLAG_BH_REFERENCE_LISTS)) {
         const NodeF32* nodes32 = B<NodeF32>(s.n32);

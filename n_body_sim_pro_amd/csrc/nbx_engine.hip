@@ -365,9 +365,8 @@ void nbx_engine_destroy(NBXEngine* engine) {
     nbx::Buf* bufs[] = {&s.b4, &s.partial, &s.flags, &s.mkey,
                         &s.tkey, &s.sk0, &s.sk1, &s.sv0, &s.sv1, &s.hist, &s.mperm,
                         &s.mkey_sorted, &s.tperm, &s.trank, &s.tword, &s.ext, &s.delta, &s.cnt,
-                        &s.scan, &s.scan_part, &s.sx, &s.sy, &s.sz, &s.sm, &s.s4, &s.n64, &s.n32,
-                        &s.ncx, &s.ncy, &s.ncz, &s.nm, &s.nskip, &s.nlist, &s.nstart, &s.ncount,
-                        &s.ndepth, &s.nminpos, &s.ninode, &s.nkids,
+                        &s.scan, &s.scan_part, &s.sp4, &s.s4, &s.tile_hist, &s.tile_base,
+                        &s.n64, &s.n32, &s.icom, &s.irec, &s.idepth, &s.endpos, &s.nkids,
                         &s.tmp32, &s.counters, &e->scratch, &e->scratch2};
     for (nbx::Buf* b : bufs) {
         nbx::free_buf(*b);
@@ -631,18 +630,29 @@ NBodySimProStatus nbx_engine_bh_export_nodes(NBXEngine* engine, double* rows,
     if (s.n == 0 || rows == nullptr) {
         NBX_FAIL(error, N_BODY_SIM_PRO_STATUS_INVALID_STATE, "no Barnes-Hut build to export");
     }
-    const size_t n = s.n, nodes = s.nodes;
+    const size_t n = s.n, nodes = s.nodes, internal = s.internal;
     std::vector<double> cx(nodes), cy(nodes), cz(nodes), m(nodes);
     std::vector<int32_t> depth(nodes), start(nodes), count(nodes), leaf(nodes);
     std::vector<uint64_t> word(n), ext;
     std::vector<uint32_t> trank(n);
-    NBX_HIP(error, hipMemcpy(cx.data(), s.ncx.p, nodes * 8, hipMemcpyDeviceToHost));
-    NBX_HIP(error, hipMemcpy(cy.data(), s.ncy.p, nodes * 8, hipMemcpyDeviceToHost));
-    NBX_HIP(error, hipMemcpy(cz.data(), s.ncz.p, nodes * 8, hipMemcpyDeviceToHost));
-    NBX_HIP(error, hipMemcpy(m.data(), s.nm.p, nodes * 8, hipMemcpyDeviceToHost));
-    NBX_HIP(error, hipMemcpy(depth.data(), s.ndepth.p, nodes * 4, hipMemcpyDeviceToHost));
-    NBX_HIP(error, hipMemcpy(start.data(), s.nstart.p, nodes * 4, hipMemcpyDeviceToHost));
-    NBX_HIP(error, hipMemcpy(count.data(), s.ncount.p, nodes * 4, hipMemcpyDeviceToHost));
+    {
+        // internal nodes: records by internal id, placed at their node index
+        std::vector<double> com(4 * internal);
+        std::vector<int32_t> rec(4 * internal), idepth(internal);
+        NBX_HIP(error, hipMemcpy(com.data(), s.icom.p, internal * 32, hipMemcpyDeviceToHost));
+        NBX_HIP(error, hipMemcpy(rec.data(), s.irec.p, internal * 16, hipMemcpyDeviceToHost));
+        NBX_HIP(error, hipMemcpy(idepth.data(), s.idepth.p, internal * 4, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < internal; ++i) {
+            const size_t k = static_cast<size_t>(rec[4 * i]);
+            cx[k] = com[4 * i];
+            cy[k] = com[4 * i + 1];
+            cz[k] = com[4 * i + 2];
+            m[k] = com[4 * i + 3];
+            depth[k] = idepth[i];
+            start[k] = rec[4 * i + 2];
+            count[k] = rec[4 * i + 3];
+        }
+    }
     {
         std::vector<nbx::NodeF32> recs(nodes);
         NBX_HIP(error, hipMemcpy(recs.data(), s.n32.p, nodes * sizeof(nbx::NodeF32),
@@ -651,16 +661,12 @@ NBodySimProStatus nbx_engine_bh_export_nodes(NBXEngine* engine, double* rows,
     }
     NBX_HIP(error, hipMemcpy(word.data(), s.tword.p, n * 8, hipMemcpyDeviceToHost));
     NBX_HIP(error, hipMemcpy(trank.data(), s.trank.p, n * 4, hipMemcpyDeviceToHost));
-    // The node SoA holds internal nodes only; a leaf's row comes from its
-    // tree position t: depth one below the deeper neighbour cell, the
-    // particle itself as mass and centre of mass.
+    // A leaf's row comes from its tree position t: depth one below the deeper
+    // neighbour cell, the particle itself as mass and centre of mass.
     std::vector<int32_t> delta(n);
-    std::vector<double> sx(n), sy(n), sz(n), sm(n);
+    std::vector<double> sp(4 * n);
     NBX_HIP(error, hipMemcpy(delta.data(), s.delta.p, n * 4, hipMemcpyDeviceToHost));
-    NBX_HIP(error, hipMemcpy(sx.data(), s.sx.p, n * 8, hipMemcpyDeviceToHost));
-    NBX_HIP(error, hipMemcpy(sy.data(), s.sy.p, n * 8, hipMemcpyDeviceToHost));
-    NBX_HIP(error, hipMemcpy(sz.data(), s.sz.p, n * 8, hipMemcpyDeviceToHost));
-    NBX_HIP(error, hipMemcpy(sm.data(), s.sm.p, n * 8, hipMemcpyDeviceToHost));
+    NBX_HIP(error, hipMemcpy(sp.data(), s.sp4.p, n * 32, hipMemcpyDeviceToHost));
     for (size_t k = 0; k < nodes; ++k) {
         const int t = leaf[k];
         if (t < 0) {
@@ -670,10 +676,10 @@ NBodySimProStatus nbx_engine_bh_export_nodes(NBXEngine* engine, double* rows,
         depth[k] = 1 + (a > delta[t] ? a : delta[t]);
         start[k] = t;
         count[k] = 1;
-        m[k] = sm[t];
-        cx[k] = sx[t];
-        cy[k] = sy[t];
-        cz[k] = sz[t];
+        cx[k] = sp[4 * t];
+        cy[k] = sp[4 * t + 1];
+        cz[k] = sp[4 * t + 2];
+        m[k] = sp[4 * t + 3];
     }
     if (s.tie_path) {
         ext.resize(n * 6);

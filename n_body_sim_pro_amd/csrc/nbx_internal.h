@@ -110,13 +110,17 @@ struct BHState {
     Buf delta;                // int32 common levels between tree positions t, t+1
     Buf cnt, scan;            // internal nodes started per tree position, exclusive scan
     Buf scan_part;            // scan block partials
-    Buf sx, sy, sz, sm;       // tree-order FP64 particle data
+    Buf sp4;                  // tree-order FP64 (x, y, z, m)
     Buf s4;                   // tree-order FP32 x, y, z, m
+    Buf tile_hist, tile_base; // per-tile, per-depth internal-node counts and their scan
     // nodes (pre-order, children in descending octant order)
     Buf n64;                  // NodeF64
     Buf n32;                  // NodeF32
-    Buf ncx, ncy, ncz, nm;    // FP64 centre of mass and mass per node
-    Buf nskip, nstart, ncount, ndepth, nminpos, nlist, ninode, nkids;
+    // internal cells by level index (depth-major, tree order within a depth):
+    // FP64 centre of mass and mass (double4), (node index, skip, first tree
+    // position, particle count) (int4), depth, last tree position, 8-slot
+    // child tables
+    Buf icom, irec, idepth, endpos, nkids;
     Buf tmp32;                // scratch permutation
     Buf counters;             // unsigned long long [approx, exact]
 };

@@ -102,10 +102,19 @@ def crafted_inputs():
     # a cluster + far outlier (deep chains of single-child cells)
     pos = np.concatenate([rng.normal(scale=1e-4, size=(500, 3)), [[100.0, -50.0, 20.0]]])
     out["outlier"] = (pos, np.ones(len(pos)))
+    # massless particles: a cell whose particles all have zero mass keeps the
+    # position of its lowest-Morton-rank particle; with no mass at all the
+    # root keeps particle 0's (barnes_hut.c:230-250, :504-512)
+    pos = rng.uniform(-1, 1, size=(800, 3))
+    m = rng.uniform(0.5, 1.0, len(pos))
+    m[pos[:, 0] < -0.3] = 0.0
+    out["massless_region"] = (pos, m)
+    out["massless_all"] = (rng.uniform(-1, 1, size=(300, 3)), np.zeros(300))
     return out
 
 
-@pytest.mark.parametrize("case", ["lattice", "close_pairs", "outlier"])
+@pytest.mark.parametrize("case", ["lattice", "close_pairs", "outlier", "massless_region",
+                                  "massless_all"])
 @pytest.mark.parametrize("theta", [0.3, 0.7])
 def test_bh_fp64_crafted_inputs_bit_identical(nbx, oracle, case, theta):
     pos, m = crafted_inputs()[case]
