@@ -18,29 +18,33 @@
 // Nodes are laid out in depth-first pre-order with children in DESCENDING
 // octant order, built from the sorted keys with no per-level launches:
 // delta[t] = common levels of neighbours t, t+1 (in descending key order);
-// position t starts internal nodes at depths (delta[t-1], delta[t]]; a scan of
-// those counts gives every node's pre-order index, a binary search on the keys
-// gives each subtree's end (the "skip" pointer). That pre-order with children
-// 7..0 is exactly the order in which the reference's LIFO walk pops nodes
-// (children pushed 0..7, barnes_hut.c:617-634), so a walk that follows skip
-// pointers visits, for every query, the same nodes in the same order.
+// position t starts the cells of depths (delta[t-1], delta[t]] and ends those
+// of depths (delta[t], delta[t-1]]. A scan of the start counts gives every
+// node's pre-order index; cells are also numbered depth-major ("level
+// index"), and the k-th start and k-th end of one depth belong to the same
+// cell, which gives each cell's last position (its skip pointer) and parent
+// without searching. That pre-order with children 7..0 is exactly the order in
+// which the reference's LIFO walk pops nodes (children pushed 0..7,
+// barnes_hut.c:617-634), so a walk that follows skip pointers visits, for
+// every query, the same nodes in the same order.
 //
 // Centre of mass. Computed in FP64 from the children in octant order 0..7
-// (barnes_hut.c:230-244), one thread per internal node, deepest level first:
-// the nodes are bucketed by depth during the fill, each large level is one
+// (barnes_hut.c:230-244), one thread per cell, deepest level first: each
+// depth is one contiguous range of level indices; each large level is one
 // launch and the small levels near the root run in a single one-block launch.
-// A node reads its children through an 8-slot child table written by the
+// A cell reads its children through an 8-slot child table written by the
 // fill, so results are deterministic and bit-identical to the reference.
 //
-// Walk. One wave = 64 consecutive queries in tree order; the node index is
-// wave-uniform (node records come in through scalar loads). Each lane keeps
-// "skip_until": it is active at node i iff i >= skip_until. Active lanes apply
-// the reference's own test (d^2 > 0 and 4h^2 < theta^2 d^2, unsoftened) and
-// either take the monopole/leaf term (then skip the subtree) or descend. The
-// wave moves to i+1 if any active lane descends, else to skip[i]. Every lane
-// thus sees exactly its reference interaction set, in reference order, with
-// no stack. The FP64 variant uses the reference arithmetic and is
-// bit-identical; the FP32 variant uses FP32 nodes and v_rsq_f32.
+// Walks. The FP64 walk (and the FP32 walk with reference lists): one wave =
+// 64 consecutive queries in tree order; the node index is wave-uniform (node
+// records come in through scalar loads). Each lane keeps "skip_until": it is
+// active at node i iff i >= skip_until. Active lanes apply the reference's
+// own test (d^2 > 0 and 4h^2 < theta^2 d^2, unsoftened) and either take the
+// monopole/leaf term (then skip the subtree) or descend. The wave moves to i+1
+// if any active lane descends, else to skip[i]. Every lane thus sees exactly
+// its reference interaction set, in reference order, with no stack. The FP64
+// variant uses the reference arithmetic and is bit-identical. The default FP32
+// walk is a group walk over the level-indexed cells (group_walk_kernel).
 #include "nbx_internal.h"
 
 #include <vector>
@@ -643,9 +647,9 @@ __global__ __launch_bounds__(kThreads) void ends_kernel(const int32_t* __restric
 }
 
 // Static fields of the internal cells (by level index), the walk records of
-// the leaves, and every node's slot in its parent's child table kids[parent L
-// * 8 + octant] (internal child: its level index; leaf child: -(tree position
-// + 2)). A position's cells are visited shallow to deep, so each one's parent
+// the leaves, and every node's slot in its parent's child table
+// gnode[parent L].kids[octant] (internal child: its level index; leaf child:
+// -(tree position + 2)). A position's cells are visited shallow to deep, so each one's parent
 // is the cell visited just before it: the depth-delta[t-1] cell that holds
 // t-1 and t (already started) for the shallowest, the previous one otherwise.
 __global__ __launch_bounds__(kThreads) void fill_kernel(
@@ -655,7 +659,7 @@ __global__ __launch_bounds__(kThreads) void fill_kernel(
     const double* __restrict__ root, InternalSoA in, NodeF64* __restrict__ n64,
     NodeF32* __restrict__ n32, const unsigned* __restrict__ doff,
     const unsigned* __restrict__ start_base, const int* __restrict__ endpos,
-    int* __restrict__ kids) {
+    GroupNode* __restrict__ gnode) {
     __shared__ TileRanks r;
     tile_ranks_init(r, doff);
     const double root_h = root[3];
@@ -688,7 +692,7 @@ __global__ __launch_bounds__(kThreads) void fill_kernel(
                     in.rec[L] = make_int4(idx, skip, t, e - t + 1);
                     in.depth[L] = d;
                     if (parent >= 0) {
-                        kids[parent * 8 + digit_at(w, ext, t, d - 1)] = L;
+                        gnode[parent].kids[digit_at(w, ext, t, d - 1)] = L;
                     }
                 }
                 parent = L;
@@ -698,7 +702,7 @@ __global__ __launch_bounds__(kThreads) void fill_kernel(
             const int depth = 1 + (dprev > hi ? dprev : hi);
             const int li = static_cast<int>(sc + cnt[t]) + t;
             if (depth > 0 && parent >= 0) {
-                kids[parent * 8 + digit_at(w, ext, t, depth - 1)] = -(t + 2);
+                gnode[parent].kids[digit_at(w, ext, t, depth - 1)] = -(t + 2);
             }
             const double4 q = sp4[t];
             if (n64 != nullptr) {   // FP64 records only for the exact FP64 walk
@@ -735,15 +739,15 @@ __global__ __launch_bounds__(kThreads) void fill_kernel(
 // caller's particle 0, barnes_hut.c:504-512). Children are one level deeper
 // and were finished by the previous launch (or barrier). A child is one
 // 32-byte record: the tree-order particle or the child's centre of mass.
-__device__ inline void com_node(int L, int depth, const int* __restrict__ kids,
+__device__ inline void com_node(int L, int depth, GroupNode* __restrict__ gnode,
                                 InternalSoA in, const uint32_t* __restrict__ trank,
                                 const double4* __restrict__ sp4, const double4* __restrict__ b4,
                                 double root_h, NodeF64* __restrict__ n64,
                                 NodeF32* __restrict__ n32) {
     const int4 r = in.rec[L];
     const int p = r.x;
-    const int4 k0 = reinterpret_cast<const int4*>(kids)[2 * L];
-    const int4 k1 = reinterpret_cast<const int4*>(kids)[2 * L + 1];
+    const int4 k0 = reinterpret_cast<const int4*>(gnode + L)[2];
+    const int4 k1 = reinterpret_cast<const int4*>(gnode + L)[3];
     const int kid[8] = {k0.x, k0.y, k0.z, k0.w, k1.x, k1.y, k1.z, k1.w};
     double mass = 0.0, x = 0.0, y = 0.0, z = 0.0;
 #pragma unroll
@@ -807,17 +811,20 @@ __device__ inline void com_node(int L, int depth, const int* __restrict__ kids,
     b.leaf = -1;
     b.inv_size2 = 1.f / b.size2;
     n32[p] = b;
+    float4* g = reinterpret_cast<float4*>(gnode + L);
+    g[0] = make_float4(b.cx, b.cy, b.cz, b.m);
+    g[1] = make_float4(b.size2, b.inv_size2, 0.f, 0.f);
 }
 
 // One depth level per launch (large levels): cells first .. first + count - 1.
 __global__ __launch_bounds__(kThreads) void com_level_kernel(
-    int first, int count, int depth, const int* __restrict__ kids,
+    int first, int count, int depth, GroupNode* __restrict__ gnode,
     InternalSoA in, const uint32_t* __restrict__ trank, const double4* __restrict__ sp4,
     const double4* __restrict__ b4, const double* __restrict__ root, NodeF64* __restrict__ n64,
     NodeF32* __restrict__ n32) {
     const int k = blockIdx.x * kThreads + threadIdx.x;
     if (k < count) {
-        com_node(first + k, depth, kids, in, trank, sp4, b4, root[3], n64, n32);
+        com_node(first + k, depth, gnode, in, trank, sp4, b4, root[3], n64, n32);
     }
 }
 
@@ -825,7 +832,7 @@ __global__ __launch_bounds__(kThreads) void com_level_kernel(
 // barrier orders each level's results before the next level reads them.
 __global__ __launch_bounds__(1024) void com_small_levels_kernel(
     const unsigned* __restrict__ doff,
-    const unsigned* __restrict__ dcount, int d_hi, int d_lo, const int* __restrict__ kids,
+    const unsigned* __restrict__ dcount, int d_hi, int d_lo, GroupNode* __restrict__ gnode,
     InternalSoA in, const uint32_t* __restrict__ trank, const double4* __restrict__ sp4,
     const double4* __restrict__ b4, const double* __restrict__ root, NodeF64* __restrict__ n64,
     NodeF32* __restrict__ n32) {
@@ -834,7 +841,7 @@ __global__ __launch_bounds__(1024) void com_small_levels_kernel(
         const int count = static_cast<int>(dcount[d]);
         const int first = static_cast<int>(doff[d]);
         for (int k = threadIdx.x; k < count; k += blockDim.x) {
-            com_node(first + k, d, kids, in, trank, sp4, b4, root_h, n64, n32);
+            com_node(first + k, d, gnode, in, trank, sp4, b4, root_h, n64, n32);
         }
         __syncthreads();
     }
@@ -1095,21 +1102,36 @@ __global__ __launch_bounds__(kThreads) void walk_f32_kernel(
     }
 }
 
-// Group walk (FP32). A wave takes 64 Q consecutive tree positions (Q per lane) as one group
-// and walks the tree once for the group's bounding box: a cell is accepted
-// when the reference's test (d^2 > 0 and 4h^2 < theta^2 d^2) holds for the
-// box distance d, which implies it holds for every member, so each member's
-// interaction set refines its reference set. The pre-order node array is
-// scanned 64 nodes per step (one vector load per lane); node j of the window
-// is visited unless an earlier closed node k of the window (accepted cell or
-// leaf) has skip[k] > j, found with a wave prefix-max of skip over closed
-// nodes; the next window starts at max(i + 64, that maximum). Visited closed
-// nodes are staged in LDS and evaluated by all 64 lanes without divergence.
-// A group that is not compact (64 consecutive positions of the Morton curve
-// can straddle one of its jumps) would get a box with most of the tree in
-// its near field; such a wave walks per query with reference lists instead.
-constexpr int kGroupList = 128;      // staged cells and staged leaves per wave (each)
-constexpr int kGroupCell = 1024;     // compact: the group's smallest enclosing cell holds <= 1024 Q
+// Group walk (FP32). A wave takes 64 Q consecutive tree positions (Q per
+// lane) as one group and walks the tree once for the group's bounding box: a
+// cell is accepted when the reference's test (d^2 > 0 and 4h^2 < theta^2 d^2)
+// holds for the box distance d, which implies it holds for every member, so
+// each member's interaction set refines its reference set. The walk runs on
+// the level-indexed cells: a per-wave stack of cells in LDS is popped up to 64
+// at a time (one cell per lane, independent loads), accepted cells are
+// staged, and an opened cell pushes its internal children and stages its
+// leaf children. The dependent chain is one record load per step, about the
+// tree depth in steps, instead of one per node visited in pre-order. Staged
+// cells and leaves are evaluated by all 64 lanes without divergence (leaf
+// positions are gathered into LDS just before). A group that is not compact
+// (64 consecutive positions of the Morton curve can straddle one of its
+// jumps) would get a box with most of the tree in its near field; such a wave
+// walks per query with reference lists instead, as does a wave whose stack
+// would overflow.
+constexpr int kGroupList = 128;      // staged cells per wave
+constexpr int kGroupLeaves = 128;    // staged leaves per wave
+constexpr int kGroupStack = 512;     // pending cells per wave
+#ifndef NBX_GROUP_CELL
+#define NBX_GROUP_CELL 2048
+#endif
+constexpr int kGroupCell = NBX_GROUP_CELL;   // compact: the group's smallest enclosing cell holds <= kGroupCell Q
+#ifndef NBX_GROUP_THREADS
+#define NBX_GROUP_THREADS 64
+#endif
+// One wave per workgroup: a wave's LDS is released as soon as it finishes
+// (waves of one region take very different times; measured on MI355X, 1M
+// particles: 1.45 ms with one wave per workgroup, 1.66 ms with four).
+constexpr int kGroupThreads = NBX_GROUP_THREADS;
 // Queries per lane of the group walk: one below kGroupPairFrom particles, two
 // from there on (measured on MI355X: at 1M, 1 is ~8% faster; at 10M, 2 is
 // ~20% faster, the wider box costing less than the shared traversal saves).
@@ -1159,26 +1181,29 @@ __device__ inline void wave_lds_fence() {
 }
 
 template <bool FEATURES, int Q>
-__global__ __launch_bounds__(kThreads) void group_walk_kernel(
-    const NodeF32* __restrict__ n32, int nodes, const float4* __restrict__ s4,
+__global__ __launch_bounds__(kGroupThreads) void group_walk_kernel(
+    const NodeF32* __restrict__ n32, int nodes, const GroupNode* __restrict__ gnode,
+    int internal, const float4* __restrict__ s4,
     const uint32_t* __restrict__ tperm, const uint64_t* __restrict__ tword,
     const uint64_t* __restrict__ ext, int n, float G, float eps2, float theta2,
     double* __restrict__ ax, double* __restrict__ ay,
     double* __restrict__ az, unsigned long long* __restrict__ counters,
     float* __restrict__ features) {
-    // staged lists of one wave: accepted cells (COM, mass; size^2 and
-    // 1/size^2 with FEATURES) and leaves (position, mass; tree position)
-    __shared__ float4 lcell_all[kThreads / kWave][kGroupList];
-    __shared__ float4 lleaf_all[kThreads / kWave][kGroupList];
-    __shared__ int ltag_all[kThreads / kWave][kGroupList];
+    // per wave: pending cells; staged cells (COM, mass; size^2 and 1/size^2
+    // with FEATURES); staged leaves (tree position, then position and mass)
+    __shared__ int stack_all[kGroupThreads / kWave][kGroupStack];
+    __shared__ float4 lcell_all[kGroupThreads / kWave][kGroupList];
     __shared__ float2 lsz_all[FEATURES ? kThreads / kWave : 1][FEATURES ? kGroupList : 1];
+    __shared__ int ltag_all[kGroupThreads / kWave][kGroupLeaves];
+    __shared__ float4 lleaf_all[kGroupThreads / kWave][kGroupLeaves];
     const int lane = threadIdx.x & (kWave - 1);
     const int wid = threadIdx.x / kWave;
+    int* stack = stack_all[wid];
     float4* lcell = lcell_all[wid];
-    float4* lleaf = lleaf_all[wid];
-    int* ltag = ltag_all[wid];
     float2* lsz = lsz_all[FEATURES ? wid : 0];
-    const int wave = xcd_block(blockIdx.x, gridDim.x) * (kThreads / kWave) + wid;
+    int* ltag = ltag_all[wid];
+    float4* lleaf = lleaf_all[wid];
+    const int wave = xcd_block(blockIdx.x, gridDim.x) * (kGroupThreads / kWave) + wid;
     const int t0 = wave * kWave * Q;
     if (t0 >= n) {
         return;   // whole wave beyond the particles
@@ -1220,9 +1245,130 @@ __global__ __launch_bounds__(kThreads) void group_walk_kernel(
     const int t1 = t0 + kWave * Q - 1 < n ? t0 + kWave * Q - 1 : n - 1;
     const int c = common_levels(tword, ext, t0, t1);
     const int cell_count = run_end(tword, ext, n, t1, c) - run_start(tword, ext, t0, c) + 1;
-    const bool compact = cell_count <= kGroupCell * Q;
+    bool per_query = cell_count > kGroupCell * Q;
     WalkAcc<FEATURES> acc[Q];
-    if (!compact) {
+    if (!per_query) {
+        const uint64_t lt = lane == 0 ? 0ull : ((~0ull) >> (kWave - lane));
+        int ccnt = 0, lcnt = 0;   // staged cells, leaves (wave-uniform)
+        // evaluate the staged lists for every member
+        auto flush = [&]() {
+            wave_lds_fence();
+            for (int e = lane; e < lcnt; e += kWave) {
+                lleaf[e] = s4[ltag[e]];
+            }
+            wave_lds_fence();
+            for (int e = 0; e < ccnt; ++e) {
+                const float4 p = lcell[e];
+                const float2 sz = FEATURES ? lsz[e] : make_float2(0.f, 0.f);
+#pragma unroll
+                for (int k = 0; k < Q; ++k) {
+                    const float dx = p.x - qx[k];
+                    const float dy = p.y - qy[k];
+                    const float dz = p.z - qz[k];
+                    acc[k].cell(dx, dy, dz, fmaf(dx, dx, fmaf(dy, dy, dz * dz)), p.w, sz.x, sz.y,
+                                eps2);
+                }
+            }
+            for (int e = 0; e < lcnt; ++e) {
+                const float4 p = lleaf[e];
+                const int tag = ltag[e];
+#pragma unroll
+                for (int k = 0; k < Q; ++k) {
+                    if (tag != t[k]) {
+                        acc[k].leaf(p.x - qx[k], p.y - qy[k], p.z - qz[k], p.w, eps2);
+                    }
+                }
+            }
+            ccnt = 0;
+            lcnt = 0;
+            wave_lds_fence();
+        };
+        int sp = 0;   // pending cells (wave-uniform); the root is level index 0
+        if (internal > 0) {
+            if (lane == 0) {
+                stack[0] = 0;
+            }
+            sp = 1;
+        }
+        wave_lds_fence();
+        while (sp > 0) {
+            const int take = sp < kWave ? sp : kWave;
+            const bool act = lane < take;
+            const int ref = act ? stack[sp - take + lane] : 0;
+            sp -= take;
+            wave_lds_fence();   // pops read before pushes reuse their slots
+            float4 c0 = make_float4(0.f, 0.f, 0.f, 0.f), c1 = c0;
+            int4 k0 = make_int4(-1, -1, -1, -1), k1 = k0;
+            bool accepted = false;
+            if (act) {
+                const float4* g = reinterpret_cast<const float4*>(gnode + ref);
+                c0 = g[0];
+                c1 = g[1];
+                k0 = reinterpret_cast<const int4*>(g)[2];
+                k1 = reinterpret_cast<const int4*>(g)[3];
+                const float ddx = fmaxf(0.f, fmaxf(bx0 - c0.x, c0.x - bx1));
+                const float ddy = fmaxf(0.f, fmaxf(by0 - c0.y, c0.y - by1));
+                const float ddz = fmaxf(0.f, fmaxf(bz0 - c0.z, c0.z - bz1));
+                const float d2 = fmaf(ddx, ddx, fmaf(ddy, ddy, ddz * ddz));
+                accepted = d2 > 0.f && c1.x < theta2 * d2;
+            }
+            if (ccnt > kGroupList - kWave) {
+                flush();
+            }
+            const uint64_t cbal = __ballot(accepted);
+            if (accepted) {
+                const int pos = ccnt + static_cast<int>(__popcll(cbal & lt));
+                lcell[pos] = c0;
+                if (FEATURES) {
+                    lsz[pos] = make_float2(c1.x, c1.y);
+                }
+            }
+            ccnt += static_cast<int>(__popcll(cbal));
+            // opened cells: children in octant order, one octant per round
+            // (at most 64 new leaves and 64 new cells per round)
+            const bool open = act && !accepted;
+            const int kid[8] = {k0.x, k0.y, k0.z, k0.w, k1.x, k1.y, k1.z, k1.w};
+#pragma unroll
+            for (int o = 0; o < 8; ++o) {
+                const int ch = open ? kid[o] : -1;
+                const uint64_t lb = __ballot(ch < -1);
+                const uint64_t ib = __ballot(ch >= 0);
+                if (lb != 0) {
+                    if (lcnt > kGroupLeaves - kWave) {
+                        flush();
+                    }
+                    if (ch < -1) {
+                        ltag[lcnt + static_cast<int>(__popcll(lb & lt))] = -ch - 2;
+                    }
+                    lcnt += static_cast<int>(__popcll(lb));
+                }
+                const int np = static_cast<int>(__popcll(ib));
+                if (sp + np > kGroupStack) {
+                    per_query = true;
+                } else if (np != 0) {
+                    if (ch >= 0) {
+                        stack[sp + static_cast<int>(__popcll(ib & lt))] = ch;
+                    }
+                    sp += np;
+                }
+            }
+            if (per_query) {
+                break;
+            }
+            wave_lds_fence();   // pushes visible before the next pops
+        }
+        if (!per_query && ccnt + lcnt > 0) {
+            flush();
+        }
+        if (per_query) {
+            // stack overflow: discard the partial sums and walk per query
+#pragma unroll
+            for (int k = 0; k < Q; ++k) {
+                acc[k] = WalkAcc<FEATURES>();
+            }
+        }
+    }
+    if (per_query) {
         int skip_until[Q];
 #pragma unroll
         for (int k = 0; k < Q; ++k) {
@@ -1238,87 +1384,6 @@ __global__ __launch_bounds__(kThreads) void group_walk_kernel(
                                                 skip_until[k], acc[k]);
             }
             i = cur.leaf >= 0 || __builtin_amdgcn_ballot_w64(descend) != 0 ? i + 1 : cur.skip;
-        }
-    } else {
-        const uint64_t lt = lane == 0 ? 0ull : ((~0ull) >> (kWave - lane));
-        int ccnt = 0, lcnt = 0;   // staged cells, leaves (wave-uniform)
-        int i = 0;                // window start (wave-uniform)
-        while (true) {
-            const bool more = i < nodes;
-            if (more) {
-                const int j = i + lane;
-                const bool in = j < nodes;
-                NodeF32 nd;
-                nd.leaf = -1;
-                nd.skip = 0;
-                bool closed = false, leaf = false;
-                if (in) {
-                    nd = n32[j];
-                    leaf = nd.leaf >= 0;
-                    if (leaf) {
-                        closed = true;
-                    } else {
-                        const float ddx = fmaxf(0.f, fmaxf(bx0 - nd.cx, nd.cx - bx1));
-                        const float ddy = fmaxf(0.f, fmaxf(by0 - nd.cy, nd.cy - by1));
-                        const float ddz = fmaxf(0.f, fmaxf(bz0 - nd.cz, nd.cz - bz1));
-                        const float d2 = fmaf(ddx, ddx, fmaf(ddy, ddy, ddz * ddz));
-                        closed = d2 > 0.f && nd.size2 < theta2 * d2;
-                    }
-                }
-                int window_max;
-                const int before = wave_prefix_max_exclusive(closed ? nd.skip : 0, lane, window_max);
-                const bool take = closed && before <= j;
-                const uint64_t cbal = __ballot(take && !leaf);
-                const uint64_t lbal = __ballot(take && leaf);
-                if (take && !leaf) {
-                    const int pos = ccnt + static_cast<int>(__popcll(cbal & lt));
-                    lcell[pos] = make_float4(nd.cx, nd.cy, nd.cz, nd.m);
-                    if (FEATURES) {
-                        lsz[pos] = make_float2(nd.size2, nd.inv_size2);
-                    }
-                }
-                if (take && leaf) {
-                    const int pos = lcnt + static_cast<int>(__popcll(lbal & lt));
-                    lleaf[pos] = make_float4(nd.cx, nd.cy, nd.cz, nd.m);
-                    ltag[pos] = nd.leaf;
-                }
-                ccnt += static_cast<int>(__popcll(cbal));
-                lcnt += static_cast<int>(__popcll(lbal));
-                i = window_max > i + kWave ? window_max : i + kWave;
-            }
-            if (ccnt > kGroupList - kWave || lcnt > kGroupList - kWave ||
-                (!more && ccnt + lcnt > 0)) {
-                // evaluate the staged lists for every member
-                wave_lds_fence();
-                for (int e = 0; e < ccnt; ++e) {
-                    const float4 p = lcell[e];
-                    const float2 sz = FEATURES ? lsz[e] : make_float2(0.f, 0.f);
-#pragma unroll
-                    for (int k = 0; k < Q; ++k) {
-                        const float dx = p.x - qx[k];
-                        const float dy = p.y - qy[k];
-                        const float dz = p.z - qz[k];
-                        acc[k].cell(dx, dy, dz, fmaf(dx, dx, fmaf(dy, dy, dz * dz)), p.w, sz.x,
-                                    sz.y, eps2);
-                    }
-                }
-                for (int e = 0; e < lcnt; ++e) {
-                    const float4 p = lleaf[e];
-                    const int tag = ltag[e];
-#pragma unroll
-                    for (int k = 0; k < Q; ++k) {
-                        if (tag != t[k]) {
-                            acc[k].leaf(p.x - qx[k], p.y - qy[k], p.z - qz[k], p.w, eps2);
-                        }
-                    }
-                }
-                ccnt = 0;
-                lcnt = 0;
-                wave_lds_fence();
-            }
-            if (!more) {
-                break;
-            }
         }
     }
     unsigned long long wa = 0, we = 0;
@@ -1539,8 +1604,8 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error)
     NBX_TRY(s.irec.need<int4>(internal + 1, error));
     NBX_TRY(s.idepth.need<int32_t>(internal + 1, error));
     NBX_TRY(s.endpos.need<int>(internal + 1, error));
-    NBX_TRY(s.nkids.need<int>(8 * internal + 8, error));
-    NBX_HIP(error, hipMemsetAsync(s.nkids.p, 0xFF, (8 * internal + 8) * sizeof(int), st));
+    NBX_TRY(s.gnode.need<GroupNode>(internal + 1, error));
+    NBX_HIP(error, hipMemsetAsync(s.gnode.p, 0xFF, (internal + 1) * sizeof(GroupNode), st));
     // FP64 node records feed only the exact FP64 walk
     NodeF64* n64_out = nullptr;
     s.has_n64 = false;
@@ -1561,7 +1626,7 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error)
     fill_kernel<<<nt, kThreads, 0, st>>>(B<uint64_t>(s.tword), ext_tree, B<int32_t>(s.delta),
                                          B<uint32_t>(s.cnt), B<uint32_t>(s.scan), ni, nt, sp4,
                                          root, in, n64_out, B<NodeF32>(s.n32), flags + kDOff,
-                                         start_base, B<int>(s.endpos), B<int>(s.nkids));
+                                         start_base, B<int>(s.endpos), B<GroupNode>(s.gnode));
     NBX_HIP(error, hipGetLastError());
     NBX_HIP(error, hipEventRecord(e->ev[3], st));
     std::vector<unsigned> offs(kMaxLevels, 0);
@@ -1576,7 +1641,7 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error)
         const unsigned count = hflags[kDCount + d];
         if (count > kSmallLevel) {
             com_level_kernel<<<grid_for(count, kThreads), kThreads, 0, st>>>(
-                static_cast<int>(offs[d]), static_cast<int>(count), d, B<int>(s.nkids), in,
+                static_cast<int>(offs[d]), static_cast<int>(count), d, B<GroupNode>(s.gnode), in,
                 B<uint32_t>(s.trank), sp4, b4, root, n64_out, B<NodeF32>(s.n32));
             --d;
             continue;
@@ -1586,7 +1651,7 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error)
             --lo;
         }
         com_small_levels_kernel<<<1, 1024, 0, st>>>(
-            flags + kDOff, flags + kDCount, d, lo, B<int>(s.nkids), in,
+            flags + kDOff, flags + kDCount, d, lo, B<GroupNode>(s.gnode), in,
             B<uint32_t>(s.trank), sp4, b4, root, n64_out, B<NodeF32>(s.n32));
         d = lo - 1;
     }
@@ -1627,32 +1692,34 @@ NBodySimProStatus bh_walk(Engine* e, const NBodySimProGravity* g, bool fp64_exac
     } else if (!(e->flags & NBX_FLAG_BH_REFERENCE_LISTS)) {
         const NodeF32* nodes32 = B<NodeF32>(s.n32);
         const int nn = static_cast<int>(s.nodes);
+        const GroupNode* gn = B<GroupNode>(s.gnode);
+        const int ni_cells = static_cast<int>(s.internal);
         const float Gf = static_cast<float>(g->gravitational_constant);
         const float e2 = static_cast<float>(g->softening_squared);
         const float th2 = static_cast<float>(theta2);
         unsigned long long* cnts = B<unsigned long long>(s.counters);
         auto grid_q = [&](int q) {
-            const unsigned blocks_q = grid_for(s.n, kThreads * q);
+            const unsigned blocks_q = grid_for(s.n, kGroupThreads * q);
             return (blocks_q + kXcds - 1) / kXcds * kXcds;
         };
         if (s.n < kGroupPairFrom) {
             if (features != nullptr) {
-                group_walk_kernel<true, 1><<<grid_q(1), kThreads, 0, st>>>(
-                    nodes32, nn, B<float4>(s.s4), B<uint32_t>(s.tperm), B<uint64_t>(s.tword),
+                group_walk_kernel<true, 1><<<grid_q(1), kGroupThreads, 0, st>>>(
+                    nodes32, nn, gn, ni_cells, B<float4>(s.s4), B<uint32_t>(s.tperm), B<uint64_t>(s.tword),
                     s.ext_tree, ni, Gf, e2, th2, e->ax, e->ay, e->az, cnts, features);
             } else {
-                group_walk_kernel<false, 1><<<grid_q(1), kThreads, 0, st>>>(
-                    nodes32, nn, B<float4>(s.s4), B<uint32_t>(s.tperm), B<uint64_t>(s.tword),
+                group_walk_kernel<false, 1><<<grid_q(1), kGroupThreads, 0, st>>>(
+                    nodes32, nn, gn, ni_cells, B<float4>(s.s4), B<uint32_t>(s.tperm), B<uint64_t>(s.tword),
                     s.ext_tree, ni, Gf, e2, th2, e->ax, e->ay, e->az, cnts, nullptr);
             }
         } else {
             if (features != nullptr) {
-                group_walk_kernel<true, 2><<<grid_q(2), kThreads, 0, st>>>(
-                    nodes32, nn, B<float4>(s.s4), B<uint32_t>(s.tperm), B<uint64_t>(s.tword),
+                group_walk_kernel<true, 2><<<grid_q(2), kGroupThreads, 0, st>>>(
+                    nodes32, nn, gn, ni_cells, B<float4>(s.s4), B<uint32_t>(s.tperm), B<uint64_t>(s.tword),
                     s.ext_tree, ni, Gf, e2, th2, e->ax, e->ay, e->az, cnts, features);
             } else {
-                group_walk_kernel<false, 2><<<grid_q(2), kThreads, 0, st>>>(
-                    nodes32, nn, B<float4>(s.s4), B<uint32_t>(s.tperm), B<uint64_t>(s.tword),
+                group_walk_kernel<false, 2><<<grid_q(2), kGroupThreads, 0, st>>>(
+                    nodes32, nn, gn, ni_cells, B<float4>(s.s4), B<uint32_t>(s.tperm), B<uint64_t>(s.tword),
                     s.ext_tree, ni, Gf, e2, th2, e->ax, e->ay, e->az, cnts, nullptr);
             }
         }

@@ -75,6 +75,16 @@ struct alignas(32) NodeF32 {
     float inv_size2;
 };
 
+// Internal cell by level index for the group walk: FP32 centre of mass and
+// mass, opening size^2 and its reciprocal, and the child table (internal
+// child: its level index; leaf child: -(tree position + 2); empty: -1).
+struct alignas(64) GroupNode {
+    float cx, cy, cz, m;
+    float size2, inv_size2;
+    int32_t pad[2];
+    int32_t kids[8];
+};
+
 // FP64 walk node (exact reference arithmetic).
 struct alignas(16) NodeF64 {
     double cx, cy, cz, m;
@@ -118,9 +128,9 @@ struct BHState {
     Buf n32;                  // NodeF32
     // internal cells by level index (depth-major, tree order within a depth):
     // FP64 centre of mass and mass (double4), (node index, skip, first tree
-    // position, particle count) (int4), depth, last tree position, 8-slot
-    // child tables
-    Buf icom, irec, idepth, endpos, nkids;
+    // position, particle count) (int4), depth, last tree position, GroupNode
+    // (FP32 record and child table)
+    Buf icom, irec, idepth, endpos, gnode;
     Buf tmp32;                // scratch permutation
     Buf counters;             // unsigned long long [approx, exact]
 };
