@@ -338,8 +338,10 @@ def main():
                    "n": n, "theta": args.theta, "eps": args.eps, "dt": args.dt,
                    "seed": args.seed, "algorithm": "hip_barnes_hut"},
         "bh": {"build_ms": float(np.mean(build_ms)), "walk_ms": walk,
-               "walk": "group walk (64 particles share one list accepted against their "
-                       "bounding box; each particle's list refines its reference list)",
+               "walk": "group walk (64 consecutive particles share one list of cells "
+                       "accepted against their bounding box, found with a per-wave stack "
+                       "over level-indexed cells; each particle's list refines its reference "
+                       "list)",
                "median_rel_err_vs_fp64_direct": float(np.median(err_first)),
                "p99_rel_err_vs_fp64_direct": float(np.quantile(err_first, 0.99)),
                "error_sample": f"{sample.size} seeded particles, FP64 direct sum in the "
@@ -351,7 +353,9 @@ def main():
                    "p99_rel_err_vs_fp64_direct": float(np.quantile(err_reflist, 0.99))},
                "interactions_per_eval": float(np.mean(inter)),
                "ginteractions_s": float(np.mean(inter)) / (value * 1e-3) / 1e9,
-               "node_count": stats["node_count"], "maximum_depth": stats["maximum_depth"],
+               "node_count": stats["node_count"],
+               "internal_node_count": stats["internal_node_count"],
+               "maximum_depth": stats["maximum_depth"],
                "phase_ms_last": {k: tm[k] for k in ("bbox_keys_ms", "sort_ms", "tree_ms",
                                                     "com_ms", "walk_ms")},
                "generation_s": t_gen},

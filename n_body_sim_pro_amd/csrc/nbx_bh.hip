@@ -1132,10 +1132,8 @@ constexpr int kGroupCell = NBX_GROUP_CELL;   // compact: the group's smallest en
 // (waves of one region take very different times; measured on MI355X, 1M
 // particles: 1.45 ms with one wave per workgroup, 1.66 ms with four).
 constexpr int kGroupThreads = NBX_GROUP_THREADS;
-// Queries per lane of the group walk: one below kGroupPairFrom particles, two
-// from there on (measured on MI355X: at 1M, 1 is ~8% faster; at 10M, 2 is
-// ~20% faster, the wider box costing less than the shared traversal saves).
-constexpr size_t kGroupPairFrom = 4u << 20;
+// One query per lane in the group walk (measured on MI355X: two per lane, a
+// group of 128, is 7% slower at 10M particles and 16% slower at 1M).
 
 // Exclusive prefix max over the wave of non-negative ints, with DPP row
 // shifts and row broadcasts (no LDS round trips); `total` = max over the wave.
@@ -1292,7 +1290,12 @@ __global__ __launch_bounds__(kGroupThreads) void group_walk_kernel(
         }
         wave_lds_fence();
         while (sp > 0) {
-            const int take = sp < kWave ? sp : kWave;
+            // pop at most 64 cells, and few enough that their children (8
+            // each) fit: a wide level then spreads over more steps instead
+            // of overflowing
+            const int room = (kGroupStack - sp) / 8;
+            const int cap = room > 1 ? (room < kWave ? room : kWave) : 1;
+            const int take = sp < cap ? sp : cap;
             const bool act = lane < take;
             const int ref = act ? stack[sp - take + lane] : 0;
             sp -= take;
@@ -1702,26 +1705,16 @@ NBodySimProStatus bh_walk(Engine* e, const NBodySimProGravity* g, bool fp64_exac
             const unsigned blocks_q = grid_for(s.n, kGroupThreads * q);
             return (blocks_q + kXcds - 1) / kXcds * kXcds;
         };
-        if (s.n < kGroupPairFrom) {
-            if (features != nullptr) {
-                group_walk_kernel<true, 1><<<grid_q(1), kGroupThreads, 0, st>>>(
-                    nodes32, nn, gn, ni_cells, B<float4>(s.s4), B<uint32_t>(s.tperm), B<uint64_t>(s.tword),
-                    s.ext_tree, ni, Gf, e2, th2, e->ax, e->ay, e->az, cnts, features);
-            } else {
-                group_walk_kernel<false, 1><<<grid_q(1), kGroupThreads, 0, st>>>(
-                    nodes32, nn, gn, ni_cells, B<float4>(s.s4), B<uint32_t>(s.tperm), B<uint64_t>(s.tword),
-                    s.ext_tree, ni, Gf, e2, th2, e->ax, e->ay, e->az, cnts, nullptr);
-            }
+        if (features != nullptr) {
+            group_walk_kernel<true, 1><<<grid_q(1), kGroupThreads, 0, st>>>(
+                nodes32, nn, gn, ni_cells, B<float4>(s.s4), B<uint32_t>(s.tperm),
+                B<uint64_t>(s.tword), s.ext_tree, ni, Gf, e2, th2, e->ax, e->ay, e->az, cnts,
+                features);
         } else {
-            if (features != nullptr) {
-                group_walk_kernel<true, 2><<<grid_q(2), kGroupThreads, 0, st>>>(
-                    nodes32, nn, gn, ni_cells, B<float4>(s.s4), B<uint32_t>(s.tperm), B<uint64_t>(s.tword),
-                    s.ext_tree, ni, Gf, e2, th2, e->ax, e->ay, e->az, cnts, features);
-            } else {
-                group_walk_kernel<false, 2><<<grid_q(2), kGroupThreads, 0, st>>>(
-                    nodes32, nn, gn, ni_cells, B<float4>(s.s4), B<uint32_t>(s.tperm), B<uint64_t>(s.tword),
-                    s.ext_tree, ni, Gf, e2, th2, e->ax, e->ay, e->az, cnts, nullptr);
-            }
+            group_walk_kernel<false, 1><<<grid_q(1), kGroupThreads, 0, st>>>(
+                nodes32, nn, gn, ni_cells, B<float4>(s.s4), B<uint32_t>(s.tperm),
+                B<uint64_t>(s.tword), s.ext_tree, ni, Gf, e2, th2, e->ax, e->ay, e->az, cnts,
+                nullptr);
         }
     } else {
         const unsigned wblocks = grid_for(s.n, kThreads * kWalkQueries);
@@ -1769,10 +1762,17 @@ NBodySimProStatus barnes_hut(Engine* e, const NBodySimProGravity* g, bool fp64_e
     const double interactions = static_cast<double>(e->stats.accepted_approximations +
                                                     e->stats.exact_interactions);
     tm.kernel_flops = 20.0 * interactions;
-    // walk bytes: every query reads its position and writes 3 doubles; node
-    // records are counted once per launch (they stay in L2 across waves).
+    // walk bytes: every query reads its position (and permutation entry) and
+    // writes 3 doubles; the cell records the walk reads are counted once per
+    // launch (they stay in L2 across waves): GroupNode records for the group
+    // walk, pre-order node records for the per-query walks.
+    const bool group = !fp64_exact && !(e->flags & NBX_FLAG_BH_REFERENCE_LISTS);
+    const double node_bytes =
+        fp64_exact ? static_cast<double>(e->bh.nodes) * sizeof(NodeF64)
+        : group    ? static_cast<double>(e->bh.internal) * sizeof(GroupNode)
+                   : static_cast<double>(e->bh.nodes) * sizeof(NodeF32);
     tm.kernel_bytes = static_cast<double>(e->bh.n) * ((fp64_exact ? 24.0 : 16.0) + 24.0 + 4.0) +
-                      static_cast<double>(e->bh.nodes) * (fp64_exact ? sizeof(NodeF64) : sizeof(NodeF32));
+                      node_bytes;
     e->timings = tm;
     e->stats.build_time_seconds = t[4] * 1e-3;
     e->stats.evaluation_time_seconds = tm.walk_ms * 1e-3;

@@ -159,7 +159,7 @@ def test_bh_two_galaxy_fp32_inputs_keys_order_topology(nbx, oracle):
     assert abs((s_t[3] + s_t[4]) - (o_t[3] + o_t[4])) <= 1e-3 * (o_t[3] + o_t[4])
 
 
-@pytest.mark.parametrize("n", [200_000, 4_500_000])  # one and two queries per lane
+@pytest.mark.parametrize("n", [200_000, 4_500_000])
 def test_group_walk_refines_reference_lists(nbx, oracle, n):
     """The default FP32 group walk accepts a cell only when the whole group's
     box passes the reference test, so it makes at least the reference's
