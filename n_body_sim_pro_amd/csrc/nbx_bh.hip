@@ -960,6 +960,30 @@ struct WalkAcc {
         ++exact;
     }
 
+    // a staged leaf that may be the query itself: the same arithmetic as
+    // leaf() for any other particle, no contribution (and no division by a
+    // zero distance when eps2 = 0) for itself, without a branch
+    __device__ void leaf_or_self(float dx, float dy, float dz, float m, bool self, float eps2) {
+        const float r2 = self ? 1.f : fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
+        const float inv = __builtin_amdgcn_rsqf(r2);
+        const float s = ((self ? 0.f : m) * inv) * (inv * inv);
+        ax = fmaf(s, dx, ax);
+        ay = fmaf(s, dy, ay);
+        az = fmaf(s, dz, az);
+        exact += self ? 0u : 1u;
+    }
+
+    // accepted cell, monopole term only
+    __device__ void cell_softened(float dx, float dy, float dz, float m, float eps2) {
+        const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
+        const float inv = __builtin_amdgcn_rsqf(r2);
+        const float s = (m * inv) * (inv * inv);
+        ax = fmaf(s, dx, ax);
+        ay = fmaf(s, dy, ay);
+        az = fmaf(s, dz, az);
+        ++approx;
+    }
+
     // accepted cell at unsoftened squared distance d2
     __device__ void cell(float dx, float dy, float dz, float d2, float m, float size2,
                          float inv_size2, float eps2) {
@@ -1263,8 +1287,14 @@ __global__ __launch_bounds__(kGroupThreads) void group_walk_kernel(
                     const float dx = p.x - qx[k];
                     const float dy = p.y - qy[k];
                     const float dz = p.z - qz[k];
-                    acc[k].cell(dx, dy, dz, fmaf(dx, dx, fmaf(dy, dy, dz * dz)), p.w, sz.x, sz.y,
-                                eps2);
+                    if (FEATURES) {
+                        acc[k].cell(dx, dy, dz, fmaf(dx, dx, fmaf(dy, dy, dz * dz)), p.w, sz.x,
+                                    sz.y, eps2);
+                    } else {
+                        // softened distance straight from the FMA chain (the
+                        // unsoftened one only feeds the features)
+                        acc[k].cell_softened(dx, dy, dz, p.w, eps2);
+                    }
                 }
             }
             for (int e = 0; e < lcnt; ++e) {
@@ -1272,9 +1302,8 @@ __global__ __launch_bounds__(kGroupThreads) void group_walk_kernel(
                 const int tag = ltag[e];
 #pragma unroll
                 for (int k = 0; k < Q; ++k) {
-                    if (tag != t[k]) {
-                        acc[k].leaf(p.x - qx[k], p.y - qy[k], p.z - qz[k], p.w, eps2);
-                    }
+                    acc[k].leaf_or_self(p.x - qx[k], p.y - qy[k], p.z - qz[k], p.w, tag == t[k],
+                                        eps2);
                 }
             }
             ccnt = 0;
