@@ -2,33 +2,37 @@
  * nbx_learned.h - learned Barnes-Hut residual correction (C ABI, libnbx.so).
  *
  * A 3-layer MLP (16 -> 64 -> 64 -> 3, SiLU after the first two layers) maps
- * per-particle walk features to the residual (a_direct - a_BH) / |a_BH|,
+ * per-particle walk features to the residual (a_direct - a_BH) / |q|, q the
+ * walk's quadrupole force (below; the residual of a theta = 1 walk lies
+ * almost along q, so the target is a bounded, N-independent ratio),
  * expressed in the particle's local frame: e1 along a_BH, e2 along the
- * accepted-cell force perpendicular to e1 (or the coordinate axis least
- * aligned with e1 when that part vanishes), e3 = e1 x e2. Weights and
+ * quadrupole force of the accepted cells perpendicular to e1 (or the
+ * coordinate axis least aligned with e1 when that part vanishes), e3 = e1 x e2. Weights and
  * activations are BF16 on the MFMA units with FP32 accumulation; master
  * weights, gradients and Adam moments are FP32. The reference has no learned
  * component; it only counts accepted cells during its walk
  * (src/c/barnes_hut/barnes_hut.c:611), which is one of the features.
  *
  * Features (raw, before standardisation; all frame-free), per particle, with
- * W = sum over accepted cells of m edge^2 / r^6 d d^T (d from the particle to
- * the cell's centre of mass, r^2 = d.d + eps^2: the monopole truncation scale
- * of each cell spread over its direction) and a = the walk's acceleration:
+ * q = sum over accepted cells of the quadrupole force 5/2 (d.Q.d) d / r^7 -
+ * Q.d / r^5 (d from the particle to the cell's centre of mass, r^2 = d.d +
+ * eps^2, Q = sum m (3 x x^T - |x|^2 I) about the cell's centre of mass, the
+ * traceless moment the build computes for the learned path) and a = the
+ * walk's acceleration:
  *   0  rms accepted distance / cell edge     1  minimum accepted distance / edge
- *   2  log2(tr W / |a| + 2^-40)              3  log2 local density: own mass /
- *                                               (leaf cell edge)^3
- *   4  e1.W.e1 / tr W                        5  e2.W.e2 / tr W
- *   6  smallest eigenvalue of W / tr W       7  largest eigenvalue of W / tr W
- *   8  log2(1 + accepted cells)              9  log2(1 + exact leaf terms)
- *   10 accepted-cell force along e1 / |a|   11  its perpendicular part / |a|
- *   12-15 zero
+ *   2  log2 local density: own mass / (leaf cell edge)^3
+ *   3  log2(1 + accepted cells)              4  log2(1 + exact leaf terms)
+ *   5  q along e1 / |a|                      6  |q perpendicular to e1| / |a|
+ *   7  q along e1 / |q|                      8  |q perpendicular to e1| / |q|
+ *   9  log2(|q| / |a| + 2^-40)               10-15 zero
+ * The counts (3, 4) grow with N and are left out of the committed model.
  * A feature whose std is +inf standardises to 0, i.e. the model ignores it.
  * Parameter layout (float, nbx_mlp_param_count() values):
  *   W1[64][16], b1[64], W2[64][64], b2[64], W3[16][64] (rows 3..15 unused,
  *   kept zero), b3[16] (entries 3..15 unused).
  * The model standardises features as (f - mean) * float32(1 / std); its output y is
- * scaled: residual / |a_BH| = (y0 e1 + y1 e2 + y2 e3) / out_scale.
+ * scaled: residual / |q| = (y0 e1 + y1 e2 + y2 e3) / out_scale (no accepted
+ * cell: q = 0, no correction).
  */
 #ifndef NBX_LEARNED_H
 #define NBX_LEARNED_H

@@ -739,11 +739,13 @@ __global__ __launch_bounds__(kThreads) void fill_kernel(
 // caller's particle 0, barnes_hut.c:504-512). Children are one level deeper
 // and were finished by the previous launch (or barrier). A child is one
 // 32-byte record: the tree-order particle or the child's centre of mass.
+template <bool QUAD>
 __device__ inline void com_node(int L, int depth, GroupNode* __restrict__ gnode,
                                 InternalSoA in, const uint32_t* __restrict__ trank,
                                 const double4* __restrict__ sp4, const double4* __restrict__ b4,
                                 double root_h, NodeF64* __restrict__ n64,
-                                NodeF32* __restrict__ n32) {
+                                NodeF32* __restrict__ n32, double* __restrict__ im2,
+                                float4* __restrict__ gquad, float4* __restrict__ gquad_pre) {
     const int4 r = in.rec[L];
     const int p = r.x;
     const int4 k0 = reinterpret_cast<const int4*>(gnode + L)[2];
@@ -808,40 +810,92 @@ __device__ inline void com_node(int L, int depth, GroupNode* __restrict__ gnode,
     b.m = static_cast<float>(mass);
     b.size2 = static_cast<float>(4.0 * h * h);
     b.skip = r.y;
-    b.leaf = -1;
+    b.leaf = -(L + 2);   // internal: -(level index + 2)
     b.inv_size2 = 1.f / b.size2;
     n32[p] = b;
+    if (QUAD) {
+        // second moment about the centre of mass from the children's
+        // (parallel axes), FP64; the walk's traceless FP32 quadrupole
+        double xx = 0.0, yy = 0.0, zz = 0.0, xy = 0.0, xz = 0.0, yz = 0.0;
+        if (mass != 0.0) {
+#pragma unroll
+            for (int o = 0; o < 8; ++o) {
+                const int c = kid[o];
+                if (c == -1) {
+                    continue;
+                }
+                const double4 q = c < -1 ? sp4[-c - 2] : in.com[c];
+                const double dx = q.x - ox, dy = q.y - oy, dz = q.z - oz;
+                xx += q.w * dx * dx;
+                yy += q.w * dy * dy;
+                zz += q.w * dz * dz;
+                xy += q.w * dx * dy;
+                xz += q.w * dx * dz;
+                yz += q.w * dy * dz;
+                if (c >= 0) {
+                    const double* cm = im2 + 6 * static_cast<size_t>(c);
+                    xx += cm[0];
+                    yy += cm[1];
+                    zz += cm[2];
+                    xy += cm[3];
+                    xz += cm[4];
+                    yz += cm[5];
+                }
+            }
+        }
+        double* om = im2 + 6 * static_cast<size_t>(L);
+        om[0] = xx;
+        om[1] = yy;
+        om[2] = zz;
+        om[3] = xy;
+        om[4] = xz;
+        om[5] = yz;
+        const double tr = xx + yy + zz;
+        gquad[2 * L] = make_float4(static_cast<float>(3.0 * xx - tr), static_cast<float>(3.0 * yy - tr),
+                                   static_cast<float>(3.0 * zz - tr), static_cast<float>(3.0 * xy));
+        gquad[2 * L + 1] = make_float4(static_cast<float>(3.0 * xz), static_cast<float>(3.0 * yz),
+                                       b.size2, b.inv_size2);
+        // the same record by pre-order node index, for the per-query walks
+        gquad_pre[2 * p] = gquad[2 * L];
+        gquad_pre[2 * p + 1] = gquad[2 * L + 1];
+    }
     float4* g = reinterpret_cast<float4*>(gnode + L);
     g[0] = make_float4(b.cx, b.cy, b.cz, b.m);
     g[1] = make_float4(b.size2, b.inv_size2, 0.f, 0.f);
 }
 
 // One depth level per launch (large levels): cells first .. first + count - 1.
+template <bool QUAD>
 __global__ __launch_bounds__(kThreads) void com_level_kernel(
     int first, int count, int depth, GroupNode* __restrict__ gnode,
     InternalSoA in, const uint32_t* __restrict__ trank, const double4* __restrict__ sp4,
     const double4* __restrict__ b4, const double* __restrict__ root, NodeF64* __restrict__ n64,
-    NodeF32* __restrict__ n32) {
+    NodeF32* __restrict__ n32, double* __restrict__ im2, float4* __restrict__ gquad,
+    float4* __restrict__ gquad_pre) {
     const int k = blockIdx.x * kThreads + threadIdx.x;
     if (k < count) {
-        com_node(first + k, depth, gnode, in, trank, sp4, b4, root[3], n64, n32);
+        com_node<QUAD>(first + k, depth, gnode, in, trank, sp4, b4, root[3], n64, n32, im2,
+                       gquad, gquad_pre);
     }
 }
 
 // Consecutive small depth levels (deepest first) in one workgroup; the
 // barrier orders each level's results before the next level reads them.
+template <bool QUAD>
 __global__ __launch_bounds__(1024) void com_small_levels_kernel(
     const unsigned* __restrict__ doff,
     const unsigned* __restrict__ dcount, int d_hi, int d_lo, GroupNode* __restrict__ gnode,
     InternalSoA in, const uint32_t* __restrict__ trank, const double4* __restrict__ sp4,
     const double4* __restrict__ b4, const double* __restrict__ root, NodeF64* __restrict__ n64,
-    NodeF32* __restrict__ n32) {
+    NodeF32* __restrict__ n32, double* __restrict__ im2, float4* __restrict__ gquad,
+    float4* __restrict__ gquad_pre) {
     const double root_h = root[3];
     for (int d = d_hi; d >= d_lo; --d) {
         const int count = static_cast<int>(dcount[d]);
         const int first = static_cast<int>(doff[d]);
         for (int k = threadIdx.x; k < count; k += blockDim.x) {
-            com_node(first + k, d, gnode, in, trank, sp4, b4, root_h, n64, n32);
+            com_node<QUAD>(first + k, d, gnode, in, trank, sp4, b4, root_h, n64, n32, im2, gquad,
+                           gquad_pre);
         }
         __syncthreads();
     }
@@ -940,14 +994,13 @@ __device__ inline int xcd_block(int b, int grid) {
 
 // Per-query accumulation of the FP32 walks: the softened interaction terms
 // and, with FEATURES, the walk statistics of the learned correction
-// (accepted-cell force, error-weighted direction tensor W, sum and minimum of
-// (distance / cell edge)^2). G is applied once, when the result is stored.
+// (quadrupole force of the accepted cells, sum and minimum of (distance /
+// cell edge)^2). G is applied once, when the result is stored.
 template <bool FEATURES>
 struct WalkAcc {
     float ax = 0.f, ay = 0.f, az = 0.f;
     unsigned approx = 0, exact = 0;
-    float ccx = 0.f, ccy = 0.f, ccz = 0.f;
-    float wxx = 0.f, wyy = 0.f, wzz = 0.f, wxy = 0.f, wxz = 0.f, wyz = 0.f;
+    float qax = 0.f, qay = 0.f, qaz = 0.f;
     float q2_sum = 0.f, q2_min = 3.0e38f;
 
     __device__ void leaf(float dx, float dy, float dz, float m, float eps2) {
@@ -984,31 +1037,32 @@ struct WalkAcc {
         ++approx;
     }
 
-    // accepted cell at unsoftened squared distance d2
-    __device__ void cell(float dx, float dy, float dz, float d2, float m, float size2,
-                         float inv_size2, float eps2) {
-        const float r2 = d2 + eps2;
+    // accepted cell at softened squared distance r2 (the force) and
+    // unsoftened d2 (the features); with FEATURES its quadrupole record
+    // q0 = (Qxx, Qyy, Qzz, Qxy), q1 = (Qxz, Qyz, size^2, 1 / size^2),
+    // Q = sum m (3 x x^T - |x|^2 I) about the centre of mass
+    __device__ void cell(float dx, float dy, float dz, float r2, float d2, float m, float4 q0,
+                         float4 q1) {
         const float inv = __builtin_amdgcn_rsqf(r2);
-        const float s = (m * inv) * (inv * inv);
+        const float inv2 = inv * inv;
+        const float s = (m * inv) * inv2;
         ax = fmaf(s, dx, ax);
         ay = fmaf(s, dy, ay);
         az = fmaf(s, dz, az);
         ++approx;
         if (FEATURES) {
-            ccx = fmaf(s, dx, ccx);
-            ccy = fmaf(s, dy, ccy);
-            ccz = fmaf(s, dz, ccz);
-            // monopole truncation scale m size^2 / r^4 spread over the
-            // direction d / r: w = m size^2 / r^6
-            const float w = (s * size2) * (inv * inv) * inv;
-            const float wx = w * dx, wy = w * dy;
-            wxx = fmaf(wx, dx, wxx);
-            wyy = fmaf(wy, dy, wyy);
-            wzz = fmaf(w * dz, dz, wzz);
-            wxy = fmaf(wx, dy, wxy);
-            wxz = fmaf(wx, dz, wxz);
-            wyz = fmaf(wy, dz, wyz);
-            const float q2 = d2 * inv_size2;
+            // quadrupole term with d = (cell - query):
+            // G (5/2 (d.Q.d) d / r^7 - Q.d / r^5), softened r
+            const float qdx = fmaf(q0.x, dx, fmaf(q0.w, dy, q1.x * dz));
+            const float qdy = fmaf(q0.w, dx, fmaf(q0.y, dy, q1.y * dz));
+            const float qdz = fmaf(q1.x, dx, fmaf(q1.y, dy, q0.z * dz));
+            const float dqd = fmaf(dx, qdx, fmaf(dy, qdy, dz * qdz));
+            const float inv5 = (inv2 * inv2) * inv;
+            const float c = (2.5f * dqd) * inv2;
+            qax = fmaf(fmaf(c, dx, -qdx), inv5, qax);
+            qay = fmaf(fmaf(c, dy, -qdy), inv5, qay);
+            qaz = fmaf(fmaf(c, dz, -qdz), inv5, qaz);
+            const float q2 = d2 * q1.w;   // (distance / cell edge)^2
             q2_sum += q2;
             q2_min = fminf(q2_min, q2);
         }
@@ -1026,13 +1080,13 @@ struct WalkAcc {
             oay[o] = static_cast<double>(G * ay);
             oaz[o] = static_cast<double>(G * az);
         } else {
-            // total sum xyz, accepted cells; accepted-cell sum xyz, exact leaf
-            // terms; W xx yy zz xy; W xz yz, sum and minimum of (d / edge)^2
+            // total sum xyz, accepted cells; quadrupole sum xyz, exact leaf
+            // terms; sum and minimum of (d / edge)^2; zeros
             float4* f = reinterpret_cast<float4*>(features + static_cast<size_t>(t) * kWalkRecord);
             f[0] = make_float4(ax, ay, az, static_cast<float>(approx));
-            f[1] = make_float4(ccx, ccy, ccz, static_cast<float>(exact));
-            f[2] = make_float4(wxx, wyy, wzz, wxy);
-            f[3] = make_float4(wxz, wyz, q2_sum, approx > 0 ? q2_min : 0.f);
+            f[1] = make_float4(qax, qay, qaz, static_cast<float>(exact));
+            f[2] = make_float4(q2_sum, approx > 0 ? q2_min : 0.f, 0.f, 0.f);
+            f[3] = make_float4(0.f, 0.f, 0.f, 0.f);
         }
     }
 };
@@ -1043,7 +1097,7 @@ struct WalkAcc {
 template <bool FEATURES>
 __device__ inline bool walk_visit(const NodeF32& cur, int i, int t, float qx, float qy, float qz,
                                   float eps2, float theta2, int& skip_until,
-                                  WalkAcc<FEATURES>& acc) {
+                                  WalkAcc<FEATURES>& acc, float4 q0, float4 q1) {
     if (i < skip_until) {
         return false;
     }
@@ -1058,7 +1112,7 @@ __device__ inline bool walk_visit(const NodeF32& cur, int i, int t, float qx, fl
     }
     const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
     if (d2 > 0.f && cur.size2 < theta2 * d2) {
-        acc.cell(dx, dy, dz, d2, cur.m, cur.size2, cur.inv_size2, eps2);
+        acc.cell(dx, dy, dz, d2 + eps2, d2, cur.m, q0, q1);
         skip_until = cur.skip;
         return false;
     }
@@ -1079,7 +1133,8 @@ __global__ __launch_bounds__(kThreads) void walk_f32_kernel(
     const NodeF32* __restrict__ n32, int nodes, const float4* __restrict__ s4,
     const uint32_t* __restrict__ tperm, int n, float G, float eps2, float theta2,
     double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az,
-    unsigned long long* __restrict__ counters, float* __restrict__ features) {
+    unsigned long long* __restrict__ counters, float* __restrict__ features,
+    const float4* __restrict__ pquad) {
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = xcd_block(blockIdx.x, gridDim.x) * (kThreads / kWave) + threadIdx.x / kWave;
     const int base = wave * (kWave * Q);
@@ -1098,12 +1153,16 @@ __global__ __launch_bounds__(kThreads) void walk_f32_kernel(
     }
     int i = 0;
     while (i < nodes) {
-        const NodeF32 cur = n32[__builtin_amdgcn_readfirstlane(i)];
+        const int iu = __builtin_amdgcn_readfirstlane(i);
+        const NodeF32 cur = n32[iu];
+        // with FEATURES the node's quadrupole record, fetched with the node
+        const float4 q0 = FEATURES ? pquad[2 * iu] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const float4 q1 = FEATURES ? pquad[2 * iu + 1] : make_float4(0.f, 0.f, 0.f, 0.f);
         bool descend = false;
 #pragma unroll
         for (int k = 0; k < Q; ++k) {
             descend |= walk_visit<FEATURES>(cur, i, t[k], qx[k], qy[k], qz[k], eps2, theta2,
-                                            skip_until[k], acc[k]);
+                                            skip_until[k], acc[k], q0, q1);
         }
         i = cur.leaf >= 0 || __builtin_amdgcn_ballot_w64(descend) != 0 ? i + 1 : cur.skip;
     }
@@ -1210,19 +1269,25 @@ __global__ __launch_bounds__(kGroupThreads) void group_walk_kernel(
     const uint64_t* __restrict__ ext, int n, float G, float eps2, float theta2,
     double* __restrict__ ax, double* __restrict__ ay,
     double* __restrict__ az, unsigned long long* __restrict__ counters,
-    float* __restrict__ features) {
-    // per wave: pending cells; staged cells (COM, mass; size^2 and 1/size^2
-    // with FEATURES); staged leaves (tree position, then position and mass)
+    float* __restrict__ features, const float4* __restrict__ gquad,
+    const float4* __restrict__ pquad) {
+    // per wave: pending cells; staged cells (COM, mass; with FEATURES the
+    // level index, then the quadrupole record); staged leaves (tree
+    // position, then position and mass)
     __shared__ int stack_all[kGroupThreads / kWave][kGroupStack];
-    __shared__ float4 lcell_all[kGroupThreads / kWave][kGroupList];
-    __shared__ float2 lsz_all[FEATURES ? kThreads / kWave : 1][FEATURES ? kGroupList : 1];
+    // FEATURES stages fewer cells per flush: their quadrupole records take LDS
+    constexpr int kList = FEATURES ? kGroupList / 2 : kGroupList;
+    __shared__ float4 lcell_all[kGroupThreads / kWave][kList];
+    __shared__ int lref_all[FEATURES ? kGroupThreads / kWave : 1][FEATURES ? kList : 1];
+    __shared__ float4 lq_all[FEATURES ? kGroupThreads / kWave : 1][FEATURES ? 2 * kList : 1];
     __shared__ int ltag_all[kGroupThreads / kWave][kGroupLeaves];
     __shared__ float4 lleaf_all[kGroupThreads / kWave][kGroupLeaves];
     const int lane = threadIdx.x & (kWave - 1);
     const int wid = threadIdx.x / kWave;
     int* stack = stack_all[wid];
     float4* lcell = lcell_all[wid];
-    float2* lsz = lsz_all[FEATURES ? wid : 0];
+    int* lref = lref_all[FEATURES ? wid : 0];
+    float4* lq = lq_all[FEATURES ? wid : 0];
     int* ltag = ltag_all[wid];
     float4* lleaf = lleaf_all[wid];
     const int wave = xcd_block(blockIdx.x, gridDim.x) * (kGroupThreads / kWave) + wid;
@@ -1278,18 +1343,27 @@ __global__ __launch_bounds__(kGroupThreads) void group_walk_kernel(
             for (int e = lane; e < lcnt; e += kWave) {
                 lleaf[e] = s4[ltag[e]];
             }
+            if (FEATURES) {
+                for (int e = lane; e < ccnt; e += kWave) {
+                    const int L = lref[e];
+                    lq[2 * e] = gquad[2 * L];
+                    lq[2 * e + 1] = gquad[2 * L + 1];
+                }
+            }
             wave_lds_fence();
             for (int e = 0; e < ccnt; ++e) {
                 const float4 p = lcell[e];
-                const float2 sz = FEATURES ? lsz[e] : make_float2(0.f, 0.f);
+                const float4 q0 = FEATURES ? lq[2 * e] : make_float4(0.f, 0.f, 0.f, 0.f);
+                const float4 q1 = FEATURES ? lq[2 * e + 1] : make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
                 for (int k = 0; k < Q; ++k) {
                     const float dx = p.x - qx[k];
                     const float dy = p.y - qy[k];
                     const float dz = p.z - qz[k];
                     if (FEATURES) {
-                        acc[k].cell(dx, dy, dz, fmaf(dx, dx, fmaf(dy, dy, dz * dz)), p.w, sz.x,
-                                    sz.y, eps2);
+                        // the plain group walk's softened distance, bit for bit
+                        acc[k].cell(dx, dy, dz, fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2))),
+                                    fmaf(dx, dx, fmaf(dy, dy, dz * dz)), p.w, q0, q1);
                     } else {
                         // softened distance straight from the FMA chain (the
                         // unsoftened one only feeds the features)
@@ -1344,7 +1418,7 @@ __global__ __launch_bounds__(kGroupThreads) void group_walk_kernel(
                 const float d2 = fmaf(ddx, ddx, fmaf(ddy, ddy, ddz * ddz));
                 accepted = d2 > 0.f && c1.x < theta2 * d2;
             }
-            if (ccnt > kGroupList - kWave) {
+            if (ccnt > kList - kWave) {
                 flush();
             }
             const uint64_t cbal = __ballot(accepted);
@@ -1352,7 +1426,7 @@ __global__ __launch_bounds__(kGroupThreads) void group_walk_kernel(
                 const int pos = ccnt + static_cast<int>(__popcll(cbal & lt));
                 lcell[pos] = c0;
                 if (FEATURES) {
-                    lsz[pos] = make_float2(c1.x, c1.y);
+                    lref[pos] = ref;
                 }
             }
             ccnt += static_cast<int>(__popcll(cbal));
@@ -1408,12 +1482,15 @@ __global__ __launch_bounds__(kGroupThreads) void group_walk_kernel(
         }
         int i = 0;
         while (i < nodes) {
-            const NodeF32 cur = n32[__builtin_amdgcn_readfirstlane(i)];
+            const int iu = __builtin_amdgcn_readfirstlane(i);
+            const NodeF32 cur = n32[iu];
+            const float4 q0 = FEATURES ? pquad[2 * iu] : make_float4(0.f, 0.f, 0.f, 0.f);
+            const float4 q1 = FEATURES ? pquad[2 * iu + 1] : make_float4(0.f, 0.f, 0.f, 0.f);
             bool descend = false;
 #pragma unroll
             for (int k = 0; k < Q; ++k) {
                 descend |= walk_visit<FEATURES>(cur, i, t[k], qx[k], qy[k], qz[k], eps2, theta2,
-                                                skip_until[k], acc[k]);
+                                                skip_until[k], acc[k], q0, q1);
             }
             i = cur.leaf >= 0 || __builtin_amdgcn_ballot_w64(descend) != 0 ? i + 1 : cur.skip;
         }
@@ -1473,7 +1550,8 @@ constexpr int kFlagWords = 512;
 
 }  // namespace
 
-NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error) {
+NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error,
+                           bool quadrupoles) {
     BHState& s = e->bh;
     const size_t n = e->n;
     if (n == 0) {
@@ -1641,6 +1719,20 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error)
     // FP64 node records feed only the exact FP64 walk
     NodeF64* n64_out = nullptr;
     s.has_n64 = false;
+    // quadrupole records only for the learned correction's feature walk
+    double* im2 = nullptr;
+    float4* gquad = nullptr;
+    float4* gquad_pre = nullptr;
+    s.has_quad = false;
+    if (quadrupoles) {
+        NBX_TRY(s.im2.need<double>(6 * (internal + 1), error));
+        NBX_TRY(s.gquad.need<float4>(2 * (internal + 1), error));
+        NBX_TRY(s.gquad_pre.need<float4>(2 * nodes, error));
+        im2 = B<double>(s.im2);
+        gquad = B<float4>(s.gquad);
+        gquad_pre = B<float4>(s.gquad_pre);
+        s.has_quad = true;
+    }
     if (!fp32_inputs) {
         NBX_TRY(s.n64.need<NodeF64>(nodes, error));
         n64_out = B<NodeF64>(s.n64);
@@ -1672,9 +1764,17 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error)
     for (int d = deepest; d >= 0;) {
         const unsigned count = hflags[kDCount + d];
         if (count > kSmallLevel) {
-            com_level_kernel<<<grid_for(count, kThreads), kThreads, 0, st>>>(
-                static_cast<int>(offs[d]), static_cast<int>(count), d, B<GroupNode>(s.gnode), in,
-                B<uint32_t>(s.trank), sp4, b4, root, n64_out, B<NodeF32>(s.n32));
+            if (quadrupoles) {
+                com_level_kernel<true><<<grid_for(count, kThreads), kThreads, 0, st>>>(
+                    static_cast<int>(offs[d]), static_cast<int>(count), d,
+                    B<GroupNode>(s.gnode), in, B<uint32_t>(s.trank), sp4, b4, root, n64_out,
+                    B<NodeF32>(s.n32), im2, gquad, gquad_pre);
+            } else {
+                com_level_kernel<false><<<grid_for(count, kThreads), kThreads, 0, st>>>(
+                    static_cast<int>(offs[d]), static_cast<int>(count), d,
+                    B<GroupNode>(s.gnode), in, B<uint32_t>(s.trank), sp4, b4, root, n64_out,
+                    B<NodeF32>(s.n32), nullptr, nullptr, nullptr);
+            }
             --d;
             continue;
         }
@@ -1682,9 +1782,17 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error)
         while (lo - 1 >= 0 && hflags[kDCount + lo - 1] <= kSmallLevel) {
             --lo;
         }
-        com_small_levels_kernel<<<1, 1024, 0, st>>>(
-            flags + kDOff, flags + kDCount, d, lo, B<GroupNode>(s.gnode), in,
-            B<uint32_t>(s.trank), sp4, b4, root, n64_out, B<NodeF32>(s.n32));
+        if (quadrupoles) {
+            com_small_levels_kernel<true><<<1, 1024, 0, st>>>(
+                flags + kDOff, flags + kDCount, d, lo, B<GroupNode>(s.gnode), in,
+                B<uint32_t>(s.trank), sp4, b4, root, n64_out, B<NodeF32>(s.n32), im2, gquad,
+                gquad_pre);
+        } else {
+            com_small_levels_kernel<false><<<1, 1024, 0, st>>>(
+                flags + kDOff, flags + kDCount, d, lo, B<GroupNode>(s.gnode), in,
+                B<uint32_t>(s.trank), sp4, b4, root, n64_out, B<NodeF32>(s.n32), nullptr,
+                nullptr, nullptr);
+        }
         d = lo - 1;
     }
     NBX_HIP(error, hipGetLastError());
@@ -1717,6 +1825,12 @@ NBodySimProStatus bh_walk(Engine* e, const NBodySimProGravity* g, bool fp64_exac
         NBX_FAIL(error, N_BODY_SIM_PRO_STATUS_INVALID_STATE,
                  "the exact walk needs a build on FP64 inputs");
     }
+    if (features != nullptr && !s.has_quad) {
+        NBX_FAIL(error, N_BODY_SIM_PRO_STATUS_INVALID_STATE,
+                 "the feature walk needs a build with quadrupole records");
+    }
+    const float4* gq = features != nullptr ? B<float4>(s.gquad) : nullptr;
+    const float4* gqp = features != nullptr ? B<float4>(s.gquad_pre) : nullptr;
     if (fp64_exact) {
         walk_f64_kernel<<<blocks, kThreads, 0, st>>>(
             B<NodeF64>(s.n64), static_cast<int>(s.nodes), B<double4>(s.sp4), B<uint32_t>(s.tperm), ni, g->gravitational_constant,
@@ -1738,12 +1852,12 @@ NBodySimProStatus bh_walk(Engine* e, const NBodySimProGravity* g, bool fp64_exac
             group_walk_kernel<true, 1><<<grid_q(1), kGroupThreads, 0, st>>>(
                 nodes32, nn, gn, ni_cells, B<float4>(s.s4), B<uint32_t>(s.tperm),
                 B<uint64_t>(s.tword), s.ext_tree, ni, Gf, e2, th2, e->ax, e->ay, e->az, cnts,
-                features);
+                features, gq, gqp);
         } else {
             group_walk_kernel<false, 1><<<grid_q(1), kGroupThreads, 0, st>>>(
                 nodes32, nn, gn, ni_cells, B<float4>(s.s4), B<uint32_t>(s.tperm),
                 B<uint64_t>(s.tword), s.ext_tree, ni, Gf, e2, th2, e->ax, e->ay, e->az, cnts,
-                nullptr);
+                nullptr, nullptr, nullptr);
         }
     } else {
         const unsigned wblocks = grid_for(s.n, kThreads * kWalkQueries);
@@ -1753,13 +1867,13 @@ NBodySimProStatus bh_walk(Engine* e, const NBodySimProGravity* g, bool fp64_exac
                 B<NodeF32>(s.n32), static_cast<int>(s.nodes), B<float4>(s.s4),
                 B<uint32_t>(s.tperm), ni, static_cast<float>(g->gravitational_constant),
                 static_cast<float>(g->softening_squared), static_cast<float>(theta2), e->ax,
-                e->ay, e->az, B<unsigned long long>(s.counters), features);
+                e->ay, e->az, B<unsigned long long>(s.counters), features, gqp);
         } else {
             walk_f32_kernel<false, kWalkQueries><<<wgrid, kThreads, 0, st>>>(
                 B<NodeF32>(s.n32), static_cast<int>(s.nodes), B<float4>(s.s4),
                 B<uint32_t>(s.tperm), ni, static_cast<float>(g->gravitational_constant),
                 static_cast<float>(g->softening_squared), static_cast<float>(theta2), e->ax,
-                e->ay, e->az, B<unsigned long long>(s.counters), nullptr);
+                e->ay, e->az, B<unsigned long long>(s.counters), nullptr, nullptr);
         }
     }
     NBX_HIP(error, hipGetLastError());

@@ -366,7 +366,7 @@ void nbx_engine_destroy(NBXEngine* engine) {
                         &s.tkey, &s.sk0, &s.sk1, &s.sv0, &s.sv1, &s.hist, &s.mperm,
                         &s.mkey_sorted, &s.tperm, &s.trank, &s.tword, &s.ext, &s.delta, &s.cnt,
                         &s.scan, &s.scan_part, &s.sp4, &s.s4, &s.tile_hist, &s.tile_base,
-                        &s.n64, &s.n32, &s.icom, &s.irec, &s.idepth, &s.endpos, &s.gnode,
+                        &s.n64, &s.n32, &s.icom, &s.irec, &s.idepth, &s.endpos, &s.gnode, &s.im2, &s.gquad, &s.gquad_pre,
                         &s.tmp32, &s.counters, &e->scratch, &e->scratch2};
     for (nbx::Buf* b : bufs) {
         nbx::free_buf(*b);

@@ -71,7 +71,7 @@ struct alignas(32) NodeF32 {
     float cx, cy, cz, m;
     float size2;
     int32_t skip;
-    int32_t leaf;   // tree position of the leaf particle, -1 for an internal node
+    int32_t leaf;   // tree position of the leaf particle; -(level index + 2) for an internal cell
     float inv_size2;
 };
 
@@ -102,6 +102,7 @@ struct BHState {
     bool tie_path = false;
     bool mismatch_path = false;
     bool has_n64 = false;     // the last build wrote FP64 node records
+    bool has_quad = false;    // the last build wrote quadrupole records
     const uint64_t* ext_tree = nullptr;   // tree-order extension words (tie path) or null
     double root[4] = {0, 0, 0, 1};
     // build inputs (FP64 x, y, z, m): either the master arrays or FP32-rounded copies
@@ -131,6 +132,10 @@ struct BHState {
     // position, particle count) (int4), depth, last tree position, GroupNode
     // (FP32 record and child table)
     Buf icom, irec, idepth, endpos, gnode;
+    // second moments about the centre of mass (FP64, 6 per cell) and the
+    // walk's quadrupole records (2 float4 per cell, by level index and by
+    // pre-order node index), learned path only
+    Buf im2, gquad, gquad_pre;
     Buf tmp32;                // scratch permutation
     Buf counters;             // unsigned long long [approx, exact]
 };
@@ -166,7 +171,8 @@ NBodySimProStatus direct_fp32(Engine* e, const NBodySimProGravity* g, NBodySimPr
 NBodySimProStatus refresh_pos4(Engine* e, NBodySimProError* error);
 NBodySimProStatus barnes_hut(Engine* e, const NBodySimProGravity* g, bool fp64_exact,
                              NBodySimProError* error);
-NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error);
+NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error,
+                           bool quadrupoles = false);
 NBodySimProStatus bh_walk(Engine* e, const NBodySimProGravity* g, bool fp64_exact,
                           float* features, NBodySimProError* error);
 NBodySimProStatus bh_learned(Engine* e, const NBodySimProGravity* g, NBodySimProError* error);

@@ -503,19 +503,19 @@ struct NBXMlp {
 namespace nbx {
 
 // Walk record (kWalkRecord floats per tree position, G not applied):
-//   [0..2] total sum, [3] accepted cells, [4..6] accepted-cell sum,
-//   [7] exact leaf terms, [8..13] W xx yy zz xy xz yz with
-//   W = sum over accepted cells of m edge^2 / r^6 d d^T, [14] sum and
-//   [15] minimum of (distance / cell edge)^2 over accepted cells.
+//   [0..2] total sum, [3] accepted cells, [4..6] quadrupole force of the
+//   accepted cells (sum of 5/2 (d.Q.d) d / r^7 - Q.d / r^5), [7] exact leaf
+//   terms, [8] sum and [9] minimum of (distance / cell edge)^2 over accepted
+//   cells, [10..15] zero.
 
 // Local frame of one query: e1 along the walk's total acceleration, e2 along
-// the accepted-cell part perpendicular to e1, e3 = e1 x e2. Features are
+// the quadrupole force perpendicular to e1, e3 = e1 x e2. Features are
 // frame-free and the model's outputs are residual components in this frame.
 struct Frame {
     float e1[3], e2[3], e3[3];
     float a;        // |total| (G not applied)
-    float c_par;    // accepted-cell sum along e1
-    float c_perp;   // |accepted-cell sum perpendicular to e1|
+    float q_par;    // quadrupole force along e1
+    float q_perp;   // |quadrupole force perpendicular to e1|
 };
 
 __device__ inline void local_frame(const float* __restrict__ wf, Frame& fr) {
@@ -536,10 +536,10 @@ __device__ inline void local_frame(const float* __restrict__ wf, Frame& fr) {
     const float cp = cx * fr.e1[0] + cy * fr.e1[1] + cz * fr.e1[2];
     float px = cx - cp * fr.e1[0], py = cy - cp * fr.e1[1], pz = cz - cp * fr.e1[2];
     float pn = sqrtf(px * px + py * py + pz * pz);
-    fr.c_par = cp;
-    fr.c_perp = pn;
-    if (!(pn > 1e-6f * a)) {
-        // no usable perpendicular cell force: the axis least aligned with e1
+    fr.q_par = cp;
+    fr.q_perp = pn;
+    if (!(pn > 1e-7f * a)) {
+        // no usable perpendicular quadrupole force: the axis least aligned with e1
         const float ux = fabsf(fr.e1[0]), uy = fabsf(fr.e1[1]), uz = fabsf(fr.e1[2]);
         const bool kx = ux <= uy && ux <= uz;
         const bool ky = !kx && uy <= uz;
@@ -564,67 +564,26 @@ __device__ inline void local_frame(const float* __restrict__ wf, Frame& fr) {
     fr.e3[2] = fr.e1[0] * fr.e2[1] - fr.e1[1] * fr.e2[0];
 }
 
-// Smallest and largest eigenvalue of a symmetric 3x3 matrix (closed form,
-// O. K. Smith, Comm. ACM 4(4):168, 1961).
-__device__ inline void sym3_eig_min_max(float xx, float yy, float zz, float xy, float xz, float yz,
-                                        float& lmin, float& lmax) {
-    const float p1 = xy * xy + xz * xz + yz * yz;
-    const float q = (xx + yy + zz) / 3.f;
-    if (!(p1 > 0.f)) {
-        lmin = fminf(xx, fminf(yy, zz));
-        lmax = fmaxf(xx, fmaxf(yy, zz));
-        return;
-    }
-    const float ax = xx - q, ay = yy - q, az = zz - q;
-    const float p2 = ax * ax + ay * ay + az * az + 2.f * p1;
-    const float p = sqrtf(p2 / 6.f);
-    if (!(p > 0.f)) {
-        lmin = lmax = q;
-        return;
-    }
-    const float ip = __builtin_amdgcn_rcpf(p);
-    const float bxx = ax * ip, byy = ay * ip, bzz = az * ip;
-    const float bxy = xy * ip, bxz = xz * ip, byz = yz * ip;
-    const float det = bxx * (byy * bzz - byz * byz) - bxy * (bxy * bzz - byz * bxz) +
-                      bxz * (bxy * byz - byy * bxz);
-    const float r = fminf(1.f, fmaxf(-1.f, 0.5f * det));
-    const float phi = acosf(r) / 3.f;
-    lmax = q + 2.f * p * __cosf(phi);
-    lmin = q + 2.f * p * __cosf(phi + 2.0943951f);
-}
-
 // Raw features (before standardisation) from one walk record.
 __device__ inline void raw_features(const float* __restrict__ wf, const Frame& fr, float m,
                                     int leaf_depth, float log2_root_edge, float f[kF]) {
     const float cells = wf[3], leaves = wf[7];
     const float inv = fr.a > 0.f ? __builtin_amdgcn_rcpf(fr.a) : 0.f;
-    const float wxx = wf[8], wyy = wf[9], wzz = wf[10], wxy = wf[11], wxz = wf[12], wyz = wf[13];
-    const float tr = wxx + wyy + wzz;
-    const float itr = tr > 0.f ? __builtin_amdgcn_rcpf(tr) : 0.f;
-    // e^T W e for the frame axes
-    const float a0 = fr.e1[0], a1 = fr.e1[1], a2 = fr.e1[2];
-    const float b0 = fr.e2[0], b1 = fr.e2[1], b2 = fr.e2[2];
-    const float w1 = wxx * a0 * a0 + wyy * a1 * a1 + wzz * a2 * a2 +
-                     2.f * (wxy * a0 * a1 + wxz * a0 * a2 + wyz * a1 * a2);
-    const float w2 = wxx * b0 * b0 + wyy * b1 * b1 + wzz * b2 * b2 +
-                     2.f * (wxy * b0 * b1 + wxz * b0 * b2 + wyz * b1 * b2);
-    float lmin = 0.f, lmax = 0.f;
-    sym3_eig_min_max(wxx, wyy, wzz, wxy, wxz, wyz, lmin, lmax);
-    f[0] = cells > 0.f ? sqrtf(wf[14] * __builtin_amdgcn_rcpf(cells)) : 0.f;   // rms distance / edge
-    f[1] = cells > 0.f ? sqrtf(wf[15]) : 0.f;           // minimum distance / cell edge
-    f[2] = log2f(tr * inv + 0x1p-40f);                  // truncation scale / |a|
+    const float qn = sqrtf(fr.q_par * fr.q_par + fr.q_perp * fr.q_perp);
+    const float iq = qn > 0.f ? __builtin_amdgcn_rcpf(qn) : 0.f;
+    f[0] = cells > 0.f ? sqrtf(wf[8] * __builtin_amdgcn_rcpf(cells)) : 0.f;   // rms distance / edge
+    f[1] = cells > 0.f ? sqrtf(wf[9]) : 0.f;            // minimum distance / cell edge
     // own mass over the leaf cell volume: edge = 2 h_root 2^-depth
-    f[3] = log2f(m > 0.f ? m : 1e-30f) - 3.f * (log2_root_edge - static_cast<float>(leaf_depth));
-    f[4] = w1 * itr;
-    f[5] = w2 * itr;
-    f[6] = lmin * itr;
-    f[7] = lmax * itr;
-    f[8] = log2f(1.f + cells);
-    f[9] = log2f(1.f + leaves);
-    f[10] = fr.c_par * inv;
-    f[11] = fr.c_perp * inv;
+    f[2] = log2f(m > 0.f ? m : 1e-30f) - 3.f * (log2_root_edge - static_cast<float>(leaf_depth));
+    f[3] = log2f(1.f + cells);
+    f[4] = log2f(1.f + leaves);
+    f[5] = fr.q_par * inv;                               // quadrupole force along e1 / |a|
+    f[6] = fr.q_perp * inv;                              // and perpendicular to it / |a|
+    f[7] = fr.q_par * iq;                                // cosine and sine of the angle
+    f[8] = fr.q_perp * iq;                               // between q and a
+    f[9] = log2f(qn * inv + 0x1p-40f);                   // log2 |q| / |a|
 #pragma unroll
-    for (int k = 12; k < kF; ++k) {
+    for (int k = 10; k < kF; ++k) {
         f[k] = 0.f;
     }
 }
@@ -674,8 +633,9 @@ __global__ void features_kernel(const float* __restrict__ walkf, const float4* _
     }
 }
 
-// Corrected accelerations: a += |a| (y0 e1 + y1 e2 + y2 e3) / out_scale, with
-// a the walk's result. Each wave takes 64 queries per step: every lane
+// Corrected accelerations: a += G |q| (y0 e1 + y1 e2 + y2 e3) / out_scale,
+// with a the walk's result and q its quadrupole force (G not applied; no
+// accepted cell, no correction). Each wave takes 64 queries per step: every lane
 // assembles one query's features into LDS, four 16-query MFMA passes run the
 // network, and every lane applies its own query's correction.
 __global__ __launch_bounds__(kMlpThreads) void learned_apply_kernel(
@@ -735,7 +695,10 @@ __global__ __launch_bounds__(kMlpThreads) void learned_apply_kernel(
         if (valid) {
             const float y0 = ys[lane][0], y1 = ys[lane][1], y2 = ys[lane][2];
             const uint32_t o = tperm[t];
-            const double k = sqrt(bx * bx + by * by + bz * bz) / out_scale;
+            // the output is in units of the walk's quadrupole force |q|
+            const double qn = sqrt(static_cast<double>(fr.q_par) * fr.q_par +
+                                   static_cast<double>(fr.q_perp) * fr.q_perp);
+            const double k = static_cast<double>(G) * qn / out_scale;
             ax[o] = bx + k * (y0 * fr.e1[0] + y1 * fr.e2[0] + y2 * fr.e3[0]);
             ay[o] = by + k * (y0 * fr.e1[1] + y1 * fr.e2[1] + y2 * fr.e3[1]);
             az[o] = bz + k * (y0 * fr.e1[2] + y1 * fr.e2[2] + y2 * fr.e3[2]);
@@ -749,7 +712,7 @@ NBodySimProStatus bh_learned(Engine* e, const NBodySimProGravity* g, NBodySimPro
     if (m == nullptr || !m->ready) {
         return barnes_hut(e, g, false, error);   // plain BH fallback
     }
-    NBX_TRY(bh_build(e, true, error));
+    NBX_TRY(bh_build(e, true, error, true));
     BHState& s = e->bh;
     NBX_TRY(e->scratch2.need<float>(s.n * kWalkRecord, error));
     float* walkf = e->scratch2.as<float>();
@@ -1055,7 +1018,7 @@ NBodySimProStatus nbx_engine_bh_features(NBXEngine* engine, const NBodySimProGra
     }
     nbx::Engine* e = &engine->e;
     NBX_HIP(error, hipSetDevice(e->device));
-    NBX_TRY(nbx::bh_build(e, true, error));
+    NBX_TRY(nbx::bh_build(e, true, error, true));
     nbx::BHState& s = e->bh;
     NBX_TRY(e->scratch2.need<float>(s.n * nbx::kWalkRecord, error));
     NBX_TRY(e->scratch.need<float>(s.n * (nbx::kF + 9), error));

@@ -27,7 +27,7 @@ N_FEATURES = 16
 N_HIDDEN = 64
 # Inputs the model uses (include/nbx_learned.h numbering); the others get
 # std = +inf and standardise to 0.
-FEATURES = (0, 1, 2, 3, 4, 5, 6, 7)
+FEATURES = (0, 1, 2, 7, 8, 9)
 WEIGHTS = pathlib.Path(__file__).resolve().parent / "weights" / "bh_learned_theta1.npz"
 _F = C.POINTER(C.c_float)
 
@@ -156,10 +156,18 @@ def init_params(seed, features=FEATURES):
                            np.zeros(16)]).astype(np.float32)
 
 
-def residual_in_frame(acc_bh, acc_ref, frames):
-    """(a_ref - a_bh) / |a_bh| in the local frames (rows e1, e2, e3)."""
+def quad_norm(acc_bh, feats, G=1.0):
+    """|q|, the walk's quadrupole force (features 5, 6 are its components
+    along and across e1 in units of |a_bh|)."""
     a = np.linalg.norm(acc_bh, axis=1)
-    r = (np.asarray(acc_ref) - np.asarray(acc_bh)) / np.where(a > 0, a, 1.0)[:, None]
+    f = np.asarray(feats, dtype=np.float64)
+    return a * np.hypot(f[:, 5], f[:, 6])
+
+
+def residual_in_frame(acc_bh, acc_ref, frames, qn):
+    """(a_ref - a_bh) / |q| in the local frames (rows e1, e2, e3); zero where
+    the walk accepted no cell (q = 0: no correction)."""
+    r = (np.asarray(acc_ref) - np.asarray(acc_bh)) / np.where(qn > 0, qn, np.inf)[:, None]
     return np.einsum("nij,nj->ni", frames.astype(np.float64), r)
 
 
@@ -182,9 +190,11 @@ def training_set(n=65536, seeds=(1, 2, 3, 4), theta=1.0, G=1.0, eps=0.01, device
             for seed in seeds:
                 d = plummer(n, seed=seed) if kind == "plummer" else disk_galaxy(n, seed=seed)
                 f, fr, a_bh, a_ref = snapshot_pairs(eng, d["pos"], d["m"], theta, G, eps)
-                r = residual_in_frame(a_bh, a_ref, fr)
-                X.append(f)
-                Y.append(r)
+                qn = quad_norm(a_bh, f)
+                r = residual_in_frame(a_bh, a_ref, fr, qn)
+                keep = qn > 0
+                X.append(f[keep])
+                Y.append(r[keep])
                 if log:
                     err = np.linalg.norm(a_bh - a_ref, axis=1) / np.linalg.norm(a_ref, axis=1)
                     log(f"snapshot {kind} seed {seed}: median BH error {np.median(err):.3e}")
