@@ -368,8 +368,13 @@ __global__ void iota_kernel(uint32_t* __restrict__ out, int n) {
 
 // Tree order = descending key order. asc_rank[k] = Morton rank of the k-th
 // particle in ascending true order; mperm[r] = original index of Morton rank r.
+// Without asc_rank the order is the Morton order and, unless the keys_kernel
+// flagged a true-path key that differs from its Morton key (then this pass is
+// redone with asc_rank), the true keys are the sorted Morton keys: read in
+// order instead of gathered.
 __global__ void tree_order_kernel(const uint32_t* __restrict__ asc_rank,
                                   const uint32_t* __restrict__ mperm,
+                                  const uint64_t* __restrict__ mkey_sorted,
                                   const uint64_t* __restrict__ tkey_orig,
                                   const uint64_t* __restrict__ ext_asc,
                                   const double4* __restrict__ b4, int n, uint32_t* __restrict__ tperm, uint32_t* __restrict__ trank,
@@ -384,7 +389,7 @@ __global__ void tree_order_kernel(const uint32_t* __restrict__ asc_rank,
     const uint32_t o = mperm[r];
     tperm[t] = o;
     trank[t] = r;
-    tword[t] = tkey_orig[o];
+    tword[t] = asc_rank == nullptr ? mkey_sorted[r] : tkey_orig[o];
     if (ext_asc != nullptr) {
         for (int w = 0; w < kExtWords; ++w) {
             ext_tree[static_cast<size_t>(t) * kExtWords + w] =
@@ -408,7 +413,7 @@ __global__ void tree_order_kernel(const uint32_t* __restrict__ asc_rank,
 // positions: per-tile counts of starts and ends per depth, scanned over the
 // tiles, plus wave ballots inside a tile, give every start and end its rank
 // in position order, with no search and no shared global counters.
-constexpr int kTopoItems = 8;
+constexpr int kTopoItems = 4;
 constexpr int kTopoTile = kThreads * kTopoItems;
 constexpr int kTopoWaves = kThreads / 64;
 
@@ -543,15 +548,35 @@ __global__ __launch_bounds__(kThreads) void depth_scan_kernel(const unsigned* __
     }
 }
 
-// Exclusive offsets of the per-depth level-index ranges.
-__global__ void depth_offsets_kernel(const unsigned* __restrict__ dcount,
-                                     unsigned* __restrict__ doff) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        unsigned run = 0;
-        for (int d = 0; d < kMaxLevels; ++d) {
-            doff[d] = run;
-            run += dcount[d];
+// Exclusive offsets of the per-depth level-index ranges, into LDS, by the
+// block's first wave (depth pairs per lane, one wave scan). Callers
+// synchronise before reading them.
+static_assert(kMaxLevels <= 2 * 64 + 1, "depth offsets: two depths per lane plus one");
+__device__ inline void depth_offsets(const unsigned* __restrict__ dcount, unsigned* doff) {
+    if (threadIdx.x >= 64) {
+        return;
+    }
+    const int l = threadIdx.x;
+    const unsigned v0 = 2 * l < kMaxLevels ? dcount[2 * l] : 0u;
+    const unsigned v1 = 2 * l + 1 < kMaxLevels ? dcount[2 * l + 1] : 0u;
+    const unsigned pair = v0 + v1;
+    unsigned incl = pair;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const unsigned y = __shfl_up(incl, off, 64);
+        if (l >= off) {
+            incl += y;
         }
+    }
+    const unsigned excl = incl - pair;
+    if (2 * l < kMaxLevels) {
+        doff[2 * l] = excl;
+    }
+    if (2 * l + 1 < kMaxLevels) {
+        doff[2 * l + 1] = excl + v0;
+    }
+    if (l == 63 && 2 * 64 < kMaxLevels) {
+        doff[2 * 64] = incl;
     }
 }
 
@@ -565,11 +590,11 @@ struct TileRanks {
     unsigned doff[kMaxLevels];
 };
 
-__device__ inline void tile_ranks_init(TileRanks& r, const unsigned* __restrict__ doff) {
+__device__ inline void tile_ranks_init(TileRanks& r, const unsigned* __restrict__ dcount) {
     for (int d = threadIdx.x; d < kMaxLevels; d += kThreads) {
         r.running[d] = 0;
-        r.doff[d] = doff[d];
     }
+    depth_offsets(dcount, r.doff);
 }
 
 __device__ inline void tile_ranks_clear_chunk(TileRanks& r) {
@@ -614,13 +639,16 @@ __device__ inline unsigned tile_rank_before(const TileRanks& r, const unsigned* 
 }
 
 // endpos[L] = last tree position of cell L.
+// Each cell's child table starts empty here (-1 in all eight slots); the
+// fill writes the occupied ones.
 __global__ __launch_bounds__(kThreads) void ends_kernel(const int32_t* __restrict__ delta, int n,
                                                        int tiles,
-                                                       const unsigned* __restrict__ doff,
+                                                       const unsigned* __restrict__ dcount,
                                                        const unsigned* __restrict__ end_base,
-                                                       int* __restrict__ endpos) {
+                                                       int* __restrict__ endpos,
+                                                       GroupNode* __restrict__ gnode) {
     __shared__ TileRanks r;
-    tile_ranks_init(r, doff);
+    tile_ranks_init(r, dcount);
     for (int i = 0; i < kTopoItems; ++i) {
         tile_ranks_clear_chunk(r);
         __syncthreads();
@@ -637,7 +665,11 @@ __global__ __launch_bounds__(kThreads) void ends_kernel(const int32_t* __restric
             const bool in = has && a <= d && d <= b;
             const unsigned long long m = __ballot(in);
             if (in) {
-                endpos[r.doff[d] + tile_rank_before(r, end_base, tiles, d, m)] = e;
+                const unsigned L = r.doff[d] + tile_rank_before(r, end_base, tiles, d, m);
+                endpos[L] = e;
+                int4* kids = reinterpret_cast<int4*>(gnode + L) + 2;
+                kids[0] = make_int4(-1, -1, -1, -1);
+                kids[1] = make_int4(-1, -1, -1, -1);
             }
         }
         __syncthreads();
@@ -657,11 +689,11 @@ __global__ __launch_bounds__(kThreads) void fill_kernel(
     const int32_t* __restrict__ delta, const uint32_t* __restrict__ cnt,
     const uint32_t* __restrict__ scan, int n, int tiles, const double4* __restrict__ sp4,
     const double* __restrict__ root, InternalSoA in, NodeF64* __restrict__ n64,
-    NodeF32* __restrict__ n32, const unsigned* __restrict__ doff,
+    NodeF32* __restrict__ n32, const unsigned* __restrict__ dcount,
     const unsigned* __restrict__ start_base, const int* __restrict__ endpos,
     GroupNode* __restrict__ gnode) {
     __shared__ TileRanks r;
-    tile_ranks_init(r, doff);
+    tile_ranks_init(r, dcount);
     const double root_h = root[3];
     for (int i = 0; i < kTopoItems; ++i) {
         tile_ranks_clear_chunk(r);
@@ -883,12 +915,14 @@ __global__ __launch_bounds__(kThreads) void com_level_kernel(
 // barrier orders each level's results before the next level reads them.
 template <bool QUAD>
 __global__ __launch_bounds__(1024) void com_small_levels_kernel(
-    const unsigned* __restrict__ doff,
     const unsigned* __restrict__ dcount, int d_hi, int d_lo, GroupNode* __restrict__ gnode,
     InternalSoA in, const uint32_t* __restrict__ trank, const double4* __restrict__ sp4,
     const double4* __restrict__ b4, const double* __restrict__ root, NodeF64* __restrict__ n64,
     NodeF32* __restrict__ n32, double* __restrict__ im2, float4* __restrict__ gquad,
     float4* __restrict__ gquad_pre) {
+    __shared__ unsigned doff[kMaxLevels];
+    depth_offsets(dcount, doff);
+    __syncthreads();
     const double root_h = root[3];
     for (int d = d_hi; d >= d_lo; --d) {
         const int count = static_cast<int>(dcount[d]);
@@ -1542,10 +1576,9 @@ InternalSoA internal_soa(BHState& s) {
     return in;
 }
 
-// Device word layout of s.flags: [0..7] flags, [8..8+129) depth counts,
-// [256..256+129) depth offsets.
+// Device word layout of s.flags: [0..7] flags, [8..8+129) depth counts.
 constexpr int kDCount = 8;
-constexpr int kDOff = 256;
+constexpr int kFlagRead = 256;
 constexpr int kFlagWords = 512;
 
 }  // namespace
@@ -1619,11 +1652,12 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error,
     // Tree order, neighbour depths and per-depth node counts for a given
     // ascending true-path order (null: Morton order), then one host read of
     // the flags, counts and root cube.
-    unsigned hflags[kDOff];
+    unsigned hflags[kFlagRead];
     auto topology = [&](const uint32_t* asc_rank, const uint64_t* ext_asc, uint64_t* ext_t)
         -> NBodySimProStatus {
         tree_order_kernel<<<g, kThreads, 0, st>>>(
-            asc_rank, B<uint32_t>(s.mperm), B<uint64_t>(s.tkey), ext_asc, b4, ni,
+            asc_rank, B<uint32_t>(s.mperm), B<uint64_t>(s.mkey_sorted), B<uint64_t>(s.tkey),
+            ext_asc, b4, ni,
             B<uint32_t>(s.tperm), B<uint32_t>(s.trank), B<uint64_t>(s.tword), ext_t,
             B<double4>(s.sp4), B<float4>(s.s4));
         delta_kernel<<<nt, kThreads, 0, st>>>(B<uint64_t>(s.tword), ext_t, ni, nt,
@@ -1633,7 +1667,7 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error,
                                                            B<unsigned>(s.tile_base),
                                                            flags + kDCount);
         NBX_HIP(error, hipGetLastError());
-        NBX_HIP(error, hipMemcpyAsync(hflags, flags, kDOff * sizeof(unsigned),
+        NBX_HIP(error, hipMemcpyAsync(hflags, flags, kFlagRead * sizeof(unsigned),
                                       hipMemcpyDeviceToHost, st));
         NBX_HIP(error, hipMemcpyAsync(s.root, root, 4 * sizeof(double), hipMemcpyDeviceToHost,
                                       st));
@@ -1715,7 +1749,6 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error,
     NBX_TRY(s.idepth.need<int32_t>(internal + 1, error));
     NBX_TRY(s.endpos.need<int>(internal + 1, error));
     NBX_TRY(s.gnode.need<GroupNode>(internal + 1, error));
-    NBX_HIP(error, hipMemsetAsync(s.gnode.p, 0xFF, (internal + 1) * sizeof(GroupNode), st));
     // FP64 node records feed only the exact FP64 walk
     NodeF64* n64_out = nullptr;
     s.has_n64 = false;
@@ -1741,15 +1774,14 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error,
     NBX_TRY(s.n32.need<NodeF32>(nodes, error));
     const InternalSoA in = internal_soa(s);
     const double4* sp4 = B<double4>(s.sp4);
-    depth_offsets_kernel<<<1, 1, 0, st>>>(flags + kDCount, flags + kDOff);
     s.ext_tree = ext_tree;
     const unsigned* start_base = B<unsigned>(s.tile_base);
     const unsigned* end_base = start_base + static_cast<size_t>(kMaxLevels) * tiles;
-    ends_kernel<<<nt, kThreads, 0, st>>>(B<int32_t>(s.delta), ni, nt, flags + kDOff, end_base,
-                                         B<int>(s.endpos));
+    ends_kernel<<<nt, kThreads, 0, st>>>(B<int32_t>(s.delta), ni, nt, flags + kDCount, end_base,
+                                         B<int>(s.endpos), B<GroupNode>(s.gnode));
     fill_kernel<<<nt, kThreads, 0, st>>>(B<uint64_t>(s.tword), ext_tree, B<int32_t>(s.delta),
                                          B<uint32_t>(s.cnt), B<uint32_t>(s.scan), ni, nt, sp4,
-                                         root, in, n64_out, B<NodeF32>(s.n32), flags + kDOff,
+                                         root, in, n64_out, B<NodeF32>(s.n32), flags + kDCount,
                                          start_base, B<int>(s.endpos), B<GroupNode>(s.gnode));
     NBX_HIP(error, hipGetLastError());
     NBX_HIP(error, hipEventRecord(e->ev[3], st));
@@ -1784,12 +1816,12 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error,
         }
         if (quadrupoles) {
             com_small_levels_kernel<true><<<1, 1024, 0, st>>>(
-                flags + kDOff, flags + kDCount, d, lo, B<GroupNode>(s.gnode), in,
+                flags + kDCount, d, lo, B<GroupNode>(s.gnode), in,
                 B<uint32_t>(s.trank), sp4, b4, root, n64_out, B<NodeF32>(s.n32), im2, gquad,
                 gquad_pre);
         } else {
             com_small_levels_kernel<false><<<1, 1024, 0, st>>>(
-                flags + kDOff, flags + kDCount, d, lo, B<GroupNode>(s.gnode), in,
+                flags + kDCount, d, lo, B<GroupNode>(s.gnode), in,
                 B<uint32_t>(s.trank), sp4, b4, root, n64_out, B<NodeF32>(s.n32), nullptr,
                 nullptr, nullptr);
         }
