@@ -1306,13 +1306,12 @@ __global__ __launch_bounds__(kGroupThreads) void group_walk_kernel(
     float* __restrict__ features, const float4* __restrict__ gquad,
     const float4* __restrict__ pquad) {
     // per wave: pending cells; staged cells (COM, mass; with FEATURES the
-    // level index, then the quadrupole record); staged leaves (tree
-    // position, then position and mass)
+    // quadrupole record); staged leaves (tree position, then position and
+    // mass)
     __shared__ int stack_all[kGroupThreads / kWave][kGroupStack];
     // FEATURES stages fewer cells per flush: their quadrupole records take LDS
     constexpr int kList = FEATURES ? kGroupList / 2 : kGroupList;
     __shared__ float4 lcell_all[kGroupThreads / kWave][kList];
-    __shared__ int lref_all[FEATURES ? kGroupThreads / kWave : 1][FEATURES ? kList : 1];
     __shared__ float4 lq_all[FEATURES ? kGroupThreads / kWave : 1][FEATURES ? 2 * kList : 1];
     __shared__ int ltag_all[kGroupThreads / kWave][kGroupLeaves];
     __shared__ float4 lleaf_all[kGroupThreads / kWave][kGroupLeaves];
@@ -1320,7 +1319,6 @@ __global__ __launch_bounds__(kGroupThreads) void group_walk_kernel(
     const int wid = threadIdx.x / kWave;
     int* stack = stack_all[wid];
     float4* lcell = lcell_all[wid];
-    int* lref = lref_all[FEATURES ? wid : 0];
     float4* lq = lq_all[FEATURES ? wid : 0];
     int* ltag = ltag_all[wid];
     float4* lleaf = lleaf_all[wid];
@@ -1377,13 +1375,6 @@ __global__ __launch_bounds__(kGroupThreads) void group_walk_kernel(
             for (int e = lane; e < lcnt; e += kWave) {
                 lleaf[e] = s4[ltag[e]];
             }
-            if (FEATURES) {
-                for (int e = lane; e < ccnt; e += kWave) {
-                    const int L = lref[e];
-                    lq[2 * e] = gquad[2 * L];
-                    lq[2 * e + 1] = gquad[2 * L + 1];
-                }
-            }
             wave_lds_fence();
             for (int e = 0; e < ccnt; ++e) {
                 const float4 p = lcell[e];
@@ -1438,6 +1429,7 @@ __global__ __launch_bounds__(kGroupThreads) void group_walk_kernel(
             sp -= take;
             wave_lds_fence();   // pops read before pushes reuse their slots
             float4 c0 = make_float4(0.f, 0.f, 0.f, 0.f), c1 = c0;
+            float4 gq0 = c0, gq1 = c0;
             int4 k0 = make_int4(-1, -1, -1, -1), k1 = k0;
             bool accepted = false;
             if (act) {
@@ -1446,6 +1438,11 @@ __global__ __launch_bounds__(kGroupThreads) void group_walk_kernel(
                 c1 = g[1];
                 k0 = reinterpret_cast<const int4*>(g)[2];
                 k1 = reinterpret_cast<const int4*>(g)[3];
+                if (FEATURES) {
+                    // the quadrupole record comes with the cell (loads in flight together)
+                    gq0 = gquad[2 * ref];
+                    gq1 = gquad[2 * ref + 1];
+                }
                 const float ddx = fmaxf(0.f, fmaxf(bx0 - c0.x, c0.x - bx1));
                 const float ddy = fmaxf(0.f, fmaxf(by0 - c0.y, c0.y - by1));
                 const float ddz = fmaxf(0.f, fmaxf(bz0 - c0.z, c0.z - bz1));
@@ -1460,7 +1457,8 @@ __global__ __launch_bounds__(kGroupThreads) void group_walk_kernel(
                 const int pos = ccnt + static_cast<int>(__popcll(cbal & lt));
                 lcell[pos] = c0;
                 if (FEATURES) {
-                    lref[pos] = ref;
+                    lq[2 * pos] = gq0;
+                    lq[2 * pos + 1] = gq1;
                 }
             }
             ccnt += static_cast<int>(__popcll(cbal));
