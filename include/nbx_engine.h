@@ -92,11 +92,24 @@ typedef struct NBXTimings {
 /* Name <-> algorithm for the `--algorithm` vocabulary; -1 / NULL when unknown. */
 int nbx_algorithm_from_name(const char* name);
 const char* nbx_algorithm_name(int algorithm);
+/* Name (hipDeviceProp_t::name, NUL-terminated in `len` bytes) and compute-unit
+ * count of HIP device `device`; 0 on success, -1 without such a device. For
+ * reports (the reference prints its CPU, HeadlessRunner.cpp:110-125). */
+int nbx_device_info(int device, char* name, size_t len, int* compute_units);
 
 NBXEngine* nbx_engine_create(const NBXConfig* config, NBodySimProError* error);
 void nbx_engine_destroy(NBXEngine* engine);
 void nbx_engine_set_theta(NBXEngine* engine, double theta);
 double nbx_engine_theta(const NBXEngine* engine);
+/* Particles currently on the device (the last upload's count) and the
+ * engine's capacity (the particle system's count and capacity accessors,
+ * particle_system.h). */
+size_t nbx_engine_particle_count(const NBXEngine* engine);
+size_t nbx_engine_capacity(const NBXEngine* engine);
+/* Device memory the engine holds now: the particle arrays plus its grown
+ * work buffers (the reference's n_body_sim_pro_memory_estimate_simulation,
+ * src/c/memory/memory_estimate.c, reports the host-side counterpart). */
+size_t nbx_engine_device_bytes(const NBXEngine* engine);
 
 /* Host view -> device (positions, velocities, accelerations, masses). */
 NBodySimProStatus nbx_engine_upload(NBXEngine* engine, const NBodySimProParticleSystemView* view,

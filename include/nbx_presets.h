@@ -4,6 +4,8 @@
  * They write positions, velocities and masses of an existing view (the caller
  * owns the arrays; accelerations are left untouched). All are deterministic
  * for a given seed; the same arrays then feed both the CPU and GPU paths.
+ * Generators with a stream per particle run on the host worker threads
+ * (nbx_set_host_threads) with the same output for any thread count.
  */
 #ifndef NBX_PRESETS_H
 #define NBX_PRESETS_H
@@ -57,6 +59,68 @@ NBodySimProStatus nbx_generate_two_galaxy(NBodySimProParticleSystemView* view,
 /* One exponential disk + bulge galaxy of total mass 1 at the origin. */
 NBodySimProStatus nbx_generate_disk_galaxy(NBodySimProParticleSystemView* view,
                                            const NBXGalaxyPairParams* params, uint64_t seed);
+
+#ifndef N_BODY_SIM_PRO_GENERATION_PRESETS_H
+/* include/n_body_sim_pro/generation/presets.h:27-38 */
+typedef enum NBodySimProSimulationPreset {
+    N_BODY_SIM_PRO_PRESET_TWO_BODY,
+    N_BODY_SIM_PRO_PRESET_RANDOM_CLOUD,
+    N_BODY_SIM_PRO_PRESET_SOLAR_SYSTEM,
+    N_BODY_SIM_PRO_PRESET_OPEN_CLUSTER,
+    N_BODY_SIM_PRO_PRESET_GLOBULAR_CLUSTER,
+    N_BODY_SIM_PRO_PRESET_SPIRAL_GALAXY,
+    N_BODY_SIM_PRO_PRESET_ELLIPTICAL_GALAXY,
+    N_BODY_SIM_PRO_PRESET_GALAXY_COLLISION,
+    N_BODY_SIM_PRO_PRESET_TRIPLE_GALAXY,
+    N_BODY_SIM_PRO_PRESET_COUNT
+} NBodySimProSimulationPreset;
+#endif
+
+enum {
+    NBX_PRESET_TWO_BODY = N_BODY_SIM_PRO_PRESET_TWO_BODY,
+    NBX_PRESET_RANDOM_CLOUD = N_BODY_SIM_PRO_PRESET_RANDOM_CLOUD,
+    NBX_PRESET_SOLAR_SYSTEM = N_BODY_SIM_PRO_PRESET_SOLAR_SYSTEM,
+    NBX_PRESET_OPEN_CLUSTER = N_BODY_SIM_PRO_PRESET_OPEN_CLUSTER,
+    NBX_PRESET_GLOBULAR_CLUSTER = N_BODY_SIM_PRO_PRESET_GLOBULAR_CLUSTER,
+    NBX_PRESET_SPIRAL_GALAXY = N_BODY_SIM_PRO_PRESET_SPIRAL_GALAXY,
+    NBX_PRESET_ELLIPTICAL_GALAXY = N_BODY_SIM_PRO_PRESET_ELLIPTICAL_GALAXY,
+    NBX_PRESET_GALAXY_COLLISION = N_BODY_SIM_PRO_PRESET_GALAXY_COLLISION,
+    NBX_PRESET_TRIPLE_GALAXY = N_BODY_SIM_PRO_PRESET_TRIPLE_GALAXY,
+    NBX_PRESET_COUNT = N_BODY_SIM_PRO_PRESET_COUNT
+};
+
+/* Replaces n_body_sim_pro_preset_string (presets.c:27-51); "unknown" out of range. */
+const char* nbx_preset_name(int preset);
+/* The preset whose name is `name`, or -1 (the CLI's lookup, main.cpp:136-148). */
+int nbx_preset_from_name(const char* name);
+
+/*
+ * Replaces n_body_sim_pro_preset_generate (presets.c:387-467): the nine named
+ * presets from one sequential stream, bit-identical to the reference for the
+ * same seed. Writes view->particle_count particles (positions, velocities,
+ * masses; accelerations untouched). Errors as the reference: a zero count,
+ * two_body with a count other than 2, an unknown preset -> INVALID_ARGUMENT;
+ * solar_system with no planet -> INVALID_STATE. Where the reference leaves
+ * particles unset (the remainders of galaxy_collision's n/2 halves and
+ * triple_galaxy's n/3 thirds) they are zero-mass particles at the origin.
+ */
+NBodySimProStatus nbx_preset_generate(NBodySimProParticleSystemView* view, int preset,
+                                      uint64_t seed, NBodySimProError* error);
+
+/*
+ * Replaces n_body_sim_pro_preset_generate_parallel (presets_parallel.c:286-482):
+ * a stream per particle seeded from (seed, index), over the host worker
+ * threads; the same output for any thread count, bit-identical to the
+ * reference's run on one thread.
+ */
+NBodySimProStatus nbx_preset_generate_parallel(NBodySimProParticleSystemView* view, int preset,
+                                               uint64_t seed, NBodySimProError* error);
+
+/* Host worker threads of the generators (the reference's
+ * n_body_sim_pro_threading_set_thread_count, threading.c:15-25); 0 restores
+ * the default, all online CPUs. Not to be changed while a generator runs. */
+void nbx_set_host_threads(int threads);
+int nbx_host_threads(void);
 
 /* Cluster sizes log-uniform in [min_size, max_size] (inclusive), seeded. */
 NBodySimProStatus nbx_ensemble_sizes(size_t clusters, uint32_t min_size, uint32_t max_size,

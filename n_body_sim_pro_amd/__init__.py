@@ -8,4 +8,4 @@ from .abi import (BarnesHutStats, Diagnostics, EngineConfig, Error, Gravity, Tim
 from .engine import (ALGORITHMS, Engine, Ensemble, GalaxyPairParams, NBXError,  # noqa: F401
                      TileConfig, device_count, disk_galaxy, ensemble_clusters, ensemble_sizes,
                      force_function, galaxy_pair_defaults, gravity, load, make_view, plummer,
-                     random_cloud, two_galaxy)
+                     PRESETS, preset, random_cloud, set_host_threads, two_galaxy)

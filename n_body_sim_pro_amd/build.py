@@ -1,6 +1,6 @@
 """Builds libnbx.so in-tree for gfx950: every HIP translation unit under csrc/
 plus the C17 host sources under host/, linked into one shared library, and the
-C17 benchmark driver nbx_benchmark.
+C17 benchmark driver nbx_benchmark and the headless runner nbx_headless.
 
     python -m n_body_sim_pro_amd.build
 """
@@ -18,6 +18,7 @@ HOST = PKG / "host"
 OBJ = PKG / "_build"
 LIB = PKG / "libnbx.so"
 BENCH_EXE = PKG / "nbx_benchmark"
+HEADLESS_EXE = PKG / "nbx_headless"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("NBX_OFFLOAD_ARCH", "gfx950")
 
@@ -42,7 +43,7 @@ def build(verbose=False):
     OBJ.mkdir(exist_ok=True)
     hip_srcs = sorted(CSRC.glob("*.hip"))
     c_srcs = sorted(HOST.glob("nbx_*.c"))
-    c_srcs = [p for p in c_srcs if p.name != "nbx_benchmark.c"]
+    c_srcs = [p for p in c_srcs if p.name not in ("nbx_benchmark.c", "nbx_headless.c")]
     jobs = []
     for src in hip_srcs:
         jobs.append([HIPCC, *HIP_FLAGS, "-c", str(src), "-o", str(OBJ / (src.stem + ".o"))])
@@ -61,6 +62,10 @@ def build(verbose=False):
     bench_src = HOST / "nbx_benchmark.c"
     if bench_src.exists():
         _run(["gcc", *C_FLAGS, "-fopenmp", str(bench_src), "-o", str(BENCH_EXE), f"-L{PKG}",
+              "-lnbx", f"-Wl,-rpath,{PKG}", "-lm"])
+    headless_src = HOST / "nbx_headless.c"
+    if headless_src.exists():
+        _run(["gcc", *C_FLAGS, str(headless_src), "-o", str(HEADLESS_EXE), f"-L{PKG}",
               "-lnbx", f"-Wl,-rpath,{PKG}", "-lm"])
     return LIB
 

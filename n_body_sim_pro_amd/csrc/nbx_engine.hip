@@ -3,7 +3,9 @@
 // functions of the C ABI (include/nbx_engine.h).
 #include "nbx_internal.h"
 
+#include <array>
 #include <cmath>
+#include <cstdio>
 #include <mutex>
 #include <new>
 #include <vector>
@@ -61,6 +63,17 @@ NBodySimProStatus ensure(void** ptr, size_t* cap, size_t bytes, NBodySimProError
 namespace {
 
 constexpr int kThreads = 256;
+
+// Every grow-on-demand device buffer of an engine (Barnes-Hut build and walk
+// arrays, diagnostics scratch).
+std::array<Buf*, 38> work_buffers(Engine* e) {
+    BHState& s = e->bh;
+    return {&s.b4, &s.partial, &s.flags, &s.mkey, &s.tkey, &s.sk0, &s.sk1, &s.sv0, &s.sv1,
+            &s.hist, &s.mperm, &s.mkey_sorted, &s.tperm, &s.trank, &s.tword, &s.ext, &s.delta,
+            &s.cnt, &s.scan, &s.scan_part, &s.sp4, &s.s4, &s.tile_hist, &s.tile_base, &s.n64,
+            &s.n32, &s.icom, &s.irec, &s.idepth, &s.endpos, &s.gnode, &s.im2, &s.gquad,
+            &s.gquad_pre, &s.tmp32, &s.counters, &e->scratch, &e->scratch2};
+}
 
 void free_buf(Buf& b) {
     if (b.p != nullptr) {
@@ -277,6 +290,24 @@ int nbx_device_count(void) {
     return count;
 }
 
+int nbx_device_info(int device, char* name, size_t len, int* compute_units) {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) {
+        return -1;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
+        return -1;
+    }
+    if (name != nullptr && len > 0) {
+        std::snprintf(name, len, "%s (%s)", prop.name, prop.gcnArchName);
+    }
+    if (compute_units != nullptr) {
+        *compute_units = prop.multiProcessorCount;
+    }
+    return 0;
+}
+
 int nbx_algorithm_from_name(const char* name) {
     if (name == nullptr) {
         return -1;
@@ -361,14 +392,7 @@ void nbx_engine_destroy(NBXEngine* engine) {
         if (a != nullptr) (void)hipFree(a);
     }
     if (e->pos4 != nullptr) (void)hipFree(e->pos4);
-    nbx::BHState& s = e->bh;
-    nbx::Buf* bufs[] = {&s.b4, &s.partial, &s.flags, &s.mkey,
-                        &s.tkey, &s.sk0, &s.sk1, &s.sv0, &s.sv1, &s.hist, &s.mperm,
-                        &s.mkey_sorted, &s.tperm, &s.trank, &s.tword, &s.ext, &s.delta, &s.cnt,
-                        &s.scan, &s.scan_part, &s.sp4, &s.s4, &s.tile_hist, &s.tile_base,
-                        &s.n64, &s.n32, &s.icom, &s.irec, &s.idepth, &s.endpos, &s.gnode, &s.im2, &s.gquad, &s.gquad_pre,
-                        &s.tmp32, &s.counters, &e->scratch, &e->scratch2};
-    for (nbx::Buf* b : bufs) {
+    for (nbx::Buf* b : nbx::work_buffers(e)) {
         nbx::free_buf(*b);
     }
     if (e->mlp != nullptr) {
@@ -379,6 +403,18 @@ void nbx_engine_destroy(NBXEngine* engine) {
     }
     if (e->stream != nullptr) (void)hipStreamDestroy(e->stream);
     delete engine;
+}
+
+size_t nbx_engine_device_bytes(const NBXEngine* engine) {
+    if (engine == nullptr) {
+        return 0;
+    }
+    Engine* e = const_cast<Engine*>(&engine->e);
+    size_t bytes = e->capacity * (10 * sizeof(double) + sizeof(float4));
+    for (const nbx::Buf* b : nbx::work_buffers(e)) {
+        bytes += b->cap;
+    }
+    return bytes;
 }
 
 void nbx_engine_set_theta(NBXEngine* engine, double theta) {
@@ -422,6 +458,14 @@ NBodySimProStatus nbx_engine_upload(NBXEngine* engine, const NBodySimProParticle
     e->n = n;
     e->pos4_valid = false;
     return N_BODY_SIM_PRO_STATUS_OK;
+}
+
+size_t nbx_engine_particle_count(const NBXEngine* engine) {
+    return engine != nullptr ? engine->e.n : 0;
+}
+
+size_t nbx_engine_capacity(const NBXEngine* engine) {
+    return engine != nullptr ? engine->e.capacity : 0;
 }
 
 NBodySimProStatus nbx_engine_download(NBXEngine* engine, const NBodySimProParticleSystemView* view,

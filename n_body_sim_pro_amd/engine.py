@@ -340,6 +340,35 @@ def disk_galaxy(n, seed=42, params=None):
                      C.byref(params) if params is not None else None, C.c_uint64(seed))
 
 
+PRESETS = ("two_body", "random_cloud", "solar_system", "open_cluster", "globular_cluster",
+           "spiral_galaxy", "elliptical_galaxy", "galaxy_collision", "triple_galaxy")
+
+
+def preset(name, n, seed=42, parallel=False):
+    """The reference's named preset `name` (presets.h:27-38) with its sequential
+    generator (presets.c:387-467) or, with parallel=True, its per-particle
+    generator (presets_parallel.c:286-482); both bit-identical to the reference."""
+    lib = load()
+    idx = PRESETS.index(name) if isinstance(name, str) else int(name)
+    symbol = "nbx_preset_generate_parallel" if parallel else "nbx_preset_generate"
+    fn = getattr(lib, symbol)
+    fn.argtypes = [C.POINTER(View), C.c_int, C.c_uint64, C.POINTER(Error)]
+    fn.restype = C.c_int
+    cols = {k: np.zeros(int(n)) for k in ("px", "py", "pz", "vx", "vy", "vz", "m")}
+    v = make_view(n, **cols)
+    err = Error()
+    st = fn(C.byref(v), idx, C.c_uint64(seed), C.byref(err))
+    if st != 0:
+        raise NBXError(st, f"{symbol}({name}, {n}) failed: {err.message.decode(errors='replace')}")
+    return {"pos": np.stack([cols["px"], cols["py"], cols["pz"]], 1),
+            "vel": np.stack([cols["vx"], cols["vy"], cols["vz"]], 1), "m": cols["m"]}
+
+
+def set_host_threads(threads):
+    """Worker threads of the host generators (0: all online CPUs)."""
+    load().nbx_set_host_threads(C.c_int(int(threads)))
+
+
 def ensemble_sizes(clusters, lo=64, hi=4096, seed=42):
     out = np.zeros(int(clusters), dtype=np.uint32)
     lib = load()

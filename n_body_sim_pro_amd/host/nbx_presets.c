@@ -11,78 +11,18 @@
  */
 #include "nbx_presets.h"
 
-#include <math.h>
+#include <pthread.h>
 #include <stdlib.h>
+#include <unistd.h>
 
-#ifndef M_PI
-#define M_PI 3.14159265358979323846
-#endif
+#include "nbx_host_internal.h"
 
-typedef struct rng {
-    uint64_t s[4];
-} rng;
-
-static uint64_t splitmix(uint64_t* x) {
-    *x += 0x9E3779B97F4A7C15ull;
-    uint64_t z = *x;
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-    return z ^ (z >> 31);
-}
-
-static void rng_seed(rng* g, uint64_t seed) {
-    uint64_t x = seed;
-    for (int k = 0; k < 4; ++k) {
-        g->s[k] = splitmix(&x);
-    }
-}
-
-static uint64_t rotl64(uint64_t v, int k) {
-    return (v << k) | (v >> (64 - k));
-}
-
-static uint64_t rng_u64(rng* g) {
-    const uint64_t r = rotl64(g->s[1] * 5, 7) * 9;
-    const uint64_t t = g->s[1] << 17;
-    g->s[2] ^= g->s[0];
-    g->s[3] ^= g->s[1];
-    g->s[1] ^= g->s[2];
-    g->s[0] ^= g->s[3];
-    g->s[2] ^= t;
-    g->s[3] = rotl64(g->s[3], 45);
-    return r;
-}
-
-static double rng_unit(rng* g) {
-    return (double)(rng_u64(g) >> 11) * (1.0 / 9007199254740992.0);
-}
-
-static double rng_gauss(rng* g) {
-    double u1 = 0.0;
-    while (u1 == 0.0) {
-        u1 = rng_unit(g);
-    }
-    const double u2 = rng_unit(g);
-    return sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2);
-}
-
-static void rng_direction(rng* g, double* x, double* y, double* z) {
-    for (;;) {
-        const double a = rng_gauss(g), b = rng_gauss(g), c = rng_gauss(g);
-        const double r = sqrt(a * a + b * b + c * c);
-        if (r > 1.0e-12) {
-            *x = a / r;
-            *y = b / r;
-            *z = c / r;
-            return;
-        }
-    }
-}
+typedef nbx_rng rng;
 
 /* Per-particle stream: seeded from (seed, index). */
 static void rng_particle(rng* g, uint64_t seed, uint64_t index) {
     uint64_t x = seed ^ (0xD1B54A32D192ED03ull * (index + 1));
-    rng_seed(g, splitmix(&x));
+    nbx_rng_seed(g, nbx_splitmix(&x));
 }
 
 static int view_ok(const NBodySimProParticleSystemView* v) {
@@ -101,23 +41,79 @@ static void put(NBodySimProParticleSystemView* v, size_t i, const double p[3], c
     v->masses[i] = m;
 }
 
+/* Host worker threads for the generators; 0 means all online CPUs. */
+static int host_threads = 0;
+
+void nbx_set_host_threads(int threads) {
+    host_threads = threads > 0 ? threads : 0;
+}
+
+int nbx_host_threads(void) {
+    if (host_threads > 0) {
+        return host_threads;
+    }
+    const long c = sysconf(_SC_NPROCESSORS_ONLN);
+    return c > 0 ? (int)(c < 256 ? c : 256) : 1;
+}
+
+typedef struct RangeJob {
+    nbx_range_fn fn;
+    void* ctx;
+    size_t lo, hi;
+} RangeJob;
+
+static void* range_thread(void* arg) {
+    RangeJob* j = (RangeJob*)arg;
+    j->fn(j->ctx, j->lo, j->hi);
+    return NULL;
+}
+
+void nbx_parallel_ranges(size_t n, nbx_range_fn fn, void* ctx) {
+    enum { kMinRange = 8192, kMaxThreads = 256 };
+    size_t t = (size_t)nbx_host_threads();
+    if (t > n / kMinRange) t = n / kMinRange;
+    if (t > kMaxThreads) t = kMaxThreads;
+    if (t <= 1) {
+        fn(ctx, 0, n);
+        return;
+    }
+    pthread_t tid[kMaxThreads];
+    RangeJob job[kMaxThreads];
+    int started[kMaxThreads];
+    for (size_t k = 0; k < t; ++k) {
+        job[k] = (RangeJob){fn, ctx, n * k / t, n * (k + 1) / t};
+    }
+    for (size_t k = 1; k < t; ++k) {
+        started[k] = pthread_create(&tid[k], NULL, range_thread, &job[k]) == 0;
+        if (!started[k]) {
+            range_thread(&job[k]);   /* no thread: do the range here */
+        }
+    }
+    range_thread(&job[0]);
+    for (size_t k = 1; k < t; ++k) {
+        if (started[k]) {
+            pthread_join(tid[k], NULL);
+        }
+    }
+}
+
 NBodySimProStatus nbx_preset_random_cloud(NBodySimProParticleSystemView* v, uint64_t seed) {
     if (!view_ok(v)) {
         return N_BODY_SIM_PRO_STATUS_INVALID_ARGUMENT;
     }
     const size_t n = v->particle_count;
     rng g;
-    rng_seed(&g, seed);
+    nbx_rng_seed(&g, seed);
     const double pm = 1.0 / (double)n;
     for (size_t i = 0; i < n; ++i) {
         /* uniform point in the unit ball, then U(-0.05, 0.05) velocities */
-        const double dist = 1.0 * cbrt(rng_unit(&g));
-        double dx, dy, dz;
-        rng_direction(&g, &dx, &dy, &dz);
-        const double p[3] = {dist * dx, dist * dy, dist * dz};
+        const double dist = 1.0 * cbrt(nbx_rng_unit(&g));
+        double d[3];
+        nbx_rng_direction(&g, d);
+        const double p[3] = {dist * d[0], dist * d[1], dist * d[2]};
         double u[3];
         for (int k = 0; k < 3; ++k) {
-            u[k] = -0.05 + (0.05 - -0.05) * rng_unit(&g);
+            u[k] = -0.05 + (0.05 - -0.05) * nbx_rng_unit(&g);
         }
         put(v, i, p, u, pm);
     }
@@ -176,28 +172,45 @@ static void recentre(NBodySimProParticleSystemView* v, size_t start, size_t coun
 static void plummer_particle(rng* g, double a, double mass, double p[3], double u[3]) {
     double x;
     do {
-        x = rng_unit(g);
+        x = nbx_rng_unit(g);
     } while (x <= 1.0e-10 || x > 0.999);
     const double r = a / sqrt(pow(x, -2.0 / 3.0) - 1.0);
-    double dx, dy, dz;
-    rng_direction(g, &dx, &dy, &dz);
-    p[0] = r * dx;
-    p[1] = r * dy;
-    p[2] = r * dz;
+    double d[3];
+    nbx_rng_direction(g, d);
+    p[0] = r * d[0];
+    p[1] = r * d[1];
+    p[2] = r * d[2];
     double q;
     for (;;) {
-        q = rng_unit(g);
-        const double y = 0.1 * rng_unit(g);
+        q = nbx_rng_unit(g);
+        const double y = 0.1 * nbx_rng_unit(g);
         if (y < q * q * pow(1.0 - q * q, 3.5)) {
             break;
         }
     }
     const double vesc = sqrt(2.0 * mass / sqrt(r * r + a * a));
     const double speed = q * vesc;
-    rng_direction(g, &dx, &dy, &dz);
-    u[0] = speed * dx;
-    u[1] = speed * dy;
-    u[2] = speed * dz;
+    nbx_rng_direction(g, d);
+    u[0] = speed * d[0];
+    u[1] = speed * d[1];
+    u[2] = speed * d[2];
+}
+
+typedef struct PlummerJob {
+    NBodySimProParticleSystemView* v;
+    double a, mass, pm;
+    uint64_t seed;
+} PlummerJob;
+
+static void plummer_range(void* ctx, size_t lo, size_t hi) {
+    const PlummerJob* j = (const PlummerJob*)ctx;
+    for (size_t i = lo; i < hi; ++i) {
+        rng g;
+        rng_particle(&g, j->seed, i);
+        double p[3], u[3];
+        plummer_particle(&g, j->a, j->mass, p, u);
+        put(j->v, i, p, u, j->pm);
+    }
 }
 
 NBodySimProStatus nbx_generate_plummer(NBodySimProParticleSystemView* v, double a, double mass,
@@ -206,14 +219,8 @@ NBodySimProStatus nbx_generate_plummer(NBodySimProParticleSystemView* v, double 
         return N_BODY_SIM_PRO_STATUS_INVALID_ARGUMENT;
     }
     const size_t n = v->particle_count;
-    const double pm = mass / (double)n;
-    for (size_t i = 0; i < n; ++i) {
-        rng g;
-        rng_particle(&g, seed, i);
-        double p[3], u[3];
-        plummer_particle(&g, a, mass, p, u);
-        put(v, i, p, u, pm);
-    }
+    PlummerJob job = {v, a, mass, mass / (double)n, seed};
+    nbx_parallel_ranges(n, plummer_range, &job);
     recentre(v, 0, n);
     return N_BODY_SIM_PRO_STATUS_OK;
 }
@@ -247,44 +254,62 @@ static double enclosed(const NBXGalaxyPairParams* p, double mass, double r) {
  * Writes one galaxy of total mass `mass` into [start, start+count), centred at
  * the origin, disk in the xy plane. Bulge particles come first.
  */
-static void galaxy(NBodySimProParticleSystemView* v, size_t start, size_t count, double mass,
-                   const NBXGalaxyPairParams* p, uint64_t seed) {
-    const double pm = mass / (double)count;
-    size_t nb = (size_t)floor(p->bulge_fraction * (double)count + 0.5);
-    if (nb > count) nb = count;
-    const double mb = mass * p->bulge_fraction;
-    for (size_t k = 0; k < count; ++k) {
+typedef struct GalaxyJob {
+    NBodySimProParticleSystemView* v;
+    const NBXGalaxyPairParams* p;
+    size_t start, nb;
+    double mass, mb, pm;
+    uint64_t seed;
+} GalaxyJob;
+
+static void galaxy_range(void* ctx, size_t lo, size_t hi) {
+    const GalaxyJob* j = (const GalaxyJob*)ctx;
+    const NBXGalaxyPairParams* p = j->p;
+    for (size_t k = lo; k < hi; ++k) {
         rng g;
-        rng_particle(&g, seed, start + k);
+        rng_particle(&g, j->seed, j->start + k);
         double pos[3], vel[3];
-        if (k < nb) {
-            plummer_particle(&g, p->bulge_scale, mb, pos, vel);
+        if (k < j->nb) {
+            plummer_particle(&g, p->bulge_scale, j->mb, pos, vel);
         } else {
             /* surface density ~ exp(-R/Rd): R ~ Gamma(2, Rd); z ~ sech^2(z/z0) */
             double e1, e2;
             do {
-                e1 = rng_unit(&g);
-                e2 = rng_unit(&g);
+                e1 = nbx_rng_unit(&g);
+                e2 = nbx_rng_unit(&g);
             } while (e1 <= 0.0 || e2 <= 0.0);
             const double R = p->disk_scale_length * (-log(e1) - log(e2));
-            const double phi = 2.0 * M_PI * rng_unit(&g);
+            const double phi = 2.0 * M_PI * nbx_rng_unit(&g);
             double w;
             do {
-                w = rng_unit(&g);
+                w = nbx_rng_unit(&g);
             } while (w <= 0.0 || w >= 1.0);
             const double z = p->disk_scale_height * atanh(2.0 * w - 1.0);
             pos[0] = R * cos(phi);
             pos[1] = R * sin(phi);
             pos[2] = z;
             const double rr = sqrt(R * R + z * z);
-            const double vc = rr > 0.0 ? sqrt(enclosed(p, mass, rr) / rr) : 0.0;
+            const double vc = rr > 0.0 ? sqrt(enclosed(p, j->mass, rr) / rr) : 0.0;
             const double sig = p->velocity_dispersion * vc;
-            vel[0] = -vc * sin(phi) + sig * rng_gauss(&g);
-            vel[1] = vc * cos(phi) + sig * rng_gauss(&g);
-            vel[2] = 0.5 * sig * rng_gauss(&g);
+            vel[0] = -vc * sin(phi) + sig * nbx_rng_gauss(&g);
+            vel[1] = vc * cos(phi) + sig * nbx_rng_gauss(&g);
+            vel[2] = 0.5 * sig * nbx_rng_gauss(&g);
         }
-        put(v, start + k, pos, vel, pm);
+        put(j->v, j->start + k, pos, vel, j->pm);
     }
+}
+
+/*
+ * Writes one galaxy of total mass `mass` into [start, start+count), centred at
+ * the origin, disk in the xy plane. Bulge particles come first. Each particle
+ * has its own stream, so the worker split does not change the result.
+ */
+static void galaxy(NBodySimProParticleSystemView* v, size_t start, size_t count, double mass,
+                   const NBXGalaxyPairParams* p, uint64_t seed) {
+    size_t nb = (size_t)floor(p->bulge_fraction * (double)count + 0.5);
+    if (nb > count) nb = count;
+    GalaxyJob job = {v, p, start, nb, mass, mass * p->bulge_fraction, mass / (double)count, seed};
+    nbx_parallel_ranges(count, galaxy_range, &job);
     recentre(v, start, count);
 }
 
@@ -352,10 +377,10 @@ NBodySimProStatus nbx_ensemble_sizes(size_t clusters, uint32_t lo, uint32_t hi, 
         return N_BODY_SIM_PRO_STATUS_INVALID_ARGUMENT;
     }
     rng g;
-    rng_seed(&g, seed);
+    nbx_rng_seed(&g, seed);
     const double a = log((double)lo), b = log((double)hi + 1.0);
     for (size_t k = 0; k < clusters; ++k) {
-        double s = floor(exp(a + (b - a) * rng_unit(&g)));
+        double s = floor(exp(a + (b - a) * nbx_rng_unit(&g)));
         if (s < lo) s = lo;
         if (s > hi) s = hi;
         out[k] = (uint32_t)s;
