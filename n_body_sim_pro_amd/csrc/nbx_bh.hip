@@ -1047,17 +1047,28 @@ struct WalkAcc {
         ++exact;
     }
 
-    // a staged leaf that may be the query itself: the same arithmetic as
-    // leaf() for any other particle, no contribution (and no division by a
-    // zero distance when eps2 = 0) for itself, without a branch
-    __device__ void leaf_or_self(float dx, float dy, float dz, float m, bool self, float eps2) {
+    // Group-walk terms: the same arithmetic as leaf(), counted per flush by
+    // the caller. A staged leaf can be the query itself (dx = dy = dz = 0
+    // exactly, both positions being the same float4): with eps2 > 0 its term
+    // is s * 0 = +0 and needs no test; with eps2 = 0 leaf_or_self_term gives
+    // it no contribution (and no division by a zero distance), branch-free.
+    __device__ void leaf_term(float dx, float dy, float dz, float m, float eps2) {
+        const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
+        const float inv = __builtin_amdgcn_rsqf(r2);
+        const float s = (m * inv) * (inv * inv);
+        ax = fmaf(s, dx, ax);
+        ay = fmaf(s, dy, ay);
+        az = fmaf(s, dz, az);
+    }
+
+    __device__ void leaf_or_self_term(float dx, float dy, float dz, float m, bool self,
+                                      float eps2) {
         const float r2 = self ? 1.f : fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
         const float inv = __builtin_amdgcn_rsqf(r2);
         const float s = ((self ? 0.f : m) * inv) * (inv * inv);
         ax = fmaf(s, dx, ax);
         ay = fmaf(s, dy, ay);
         az = fmaf(s, dz, az);
-        exact += self ? 0u : 1u;
     }
 
     // accepted cell, monopole term only
@@ -1238,6 +1249,8 @@ __global__ __launch_bounds__(kThreads) void walk_f32_kernel(
 constexpr int kGroupList = 128;      // staged cells per wave
 constexpr int kGroupLeaves = 128;    // staged leaves per wave
 constexpr int kGroupStack = 512;     // pending cells per wave
+constexpr int kRangeStack = 32;      // pending sub-ranges per wave
+constexpr int kMaxSubWalks = 16;     // shared-list walks per wave
 #ifndef NBX_GROUP_CELL
 #define NBX_GROUP_CELL 2048
 #endif
@@ -1295,7 +1308,7 @@ __device__ inline void wave_lds_fence() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <bool FEATURES, int Q>
+template <bool FEATURES>
 __global__ __launch_bounds__(kGroupThreads) void group_walk_kernel(
     const NodeF32* __restrict__ n32, int nodes, const GroupNode* __restrict__ gnode,
     int internal, const float4* __restrict__ s4,
@@ -1305,16 +1318,19 @@ __global__ __launch_bounds__(kGroupThreads) void group_walk_kernel(
     double* __restrict__ az, unsigned long long* __restrict__ counters,
     float* __restrict__ features, const float4* __restrict__ gquad,
     const float4* __restrict__ pquad) {
+    constexpr int kWaves = kGroupThreads / kWave;
     // per wave: pending cells; staged cells (COM, mass; with FEATURES the
     // quadrupole record); staged leaves (tree position, then position and
-    // mass)
-    __shared__ int stack_all[kGroupThreads / kWave][kGroupStack];
+    // mass); whether each lane's own leaf was staged; pending sub-ranges
+    __shared__ int stack_all[kWaves][kGroupStack];
     // FEATURES stages fewer cells per flush: their quadrupole records take LDS
     constexpr int kList = FEATURES ? kGroupList / 2 : kGroupList;
-    __shared__ float4 lcell_all[kGroupThreads / kWave][kList];
-    __shared__ float4 lq_all[FEATURES ? kGroupThreads / kWave : 1][FEATURES ? 2 * kList : 1];
-    __shared__ int ltag_all[kGroupThreads / kWave][kGroupLeaves];
-    __shared__ float4 lleaf_all[kGroupThreads / kWave][kGroupLeaves];
+    __shared__ float4 lcell_all[kWaves][kList];
+    __shared__ float4 lq_all[FEATURES ? kWaves : 1][FEATURES ? 2 * kList : 1];
+    __shared__ int ltag_all[kWaves][kGroupLeaves];
+    __shared__ float4 lleaf_all[kWaves][kGroupLeaves];
+    __shared__ unsigned char selfhit_all[kWaves][kWave];
+    __shared__ int2 ranges_all[kWaves][kRangeStack];
     const int lane = threadIdx.x & (kWave - 1);
     const int wid = threadIdx.x / kWave;
     int* stack = stack_all[wid];
@@ -1322,53 +1338,37 @@ __global__ __launch_bounds__(kGroupThreads) void group_walk_kernel(
     float4* lq = lq_all[FEATURES ? wid : 0];
     int* ltag = ltag_all[wid];
     float4* lleaf = lleaf_all[wid];
-    const int wave = xcd_block(blockIdx.x, gridDim.x) * (kGroupThreads / kWave) + wid;
-    const int t0 = wave * kWave * Q;
+    unsigned char* selfhit = selfhit_all[wid];
+    int2* ranges = ranges_all[wid];
+    const int wave = xcd_block(blockIdx.x, gridDim.x) * kWaves + wid;
+    const int t0 = wave * kWave;
     if (t0 >= n) {
         return;   // whole wave beyond the particles
     }
-    // the group: tree positions t0 + 64k + lane, k < Q
-    int t[Q];
-    float qx[Q], qy[Q], qz[Q];
-    float bx0 = 3.0e38f, by0 = 3.0e38f, bz0 = 3.0e38f;
-    float bx1 = -3.0e38f, by1 = -3.0e38f, bz1 = -3.0e38f;
-#pragma unroll
-    for (int k = 0; k < Q; ++k) {
-        t[k] = t0 + k * kWave + lane;
-        const bool valid = t[k] < n;
-        const float4 q = s4[valid ? t[k] : n - 1];
-        qx[k] = q.x;
-        qy[k] = q.y;
-        qz[k] = q.z;
-        if (valid) {
-            bx0 = fminf(bx0, q.x);
-            by0 = fminf(by0, q.y);
-            bz0 = fminf(bz0, q.z);
-            bx1 = fmaxf(bx1, q.x);
-            by1 = fmaxf(by1, q.y);
-            bz1 = fmaxf(bz1, q.z);
-        }
-    }
-    bx0 = wave_min(bx0);
-    by0 = wave_min(by0);
-    bz0 = wave_min(bz0);
-    bx1 = wave_max(bx1);
-    by1 = wave_max(by1);
-    bz1 = wave_max(bz1);
-    // Compactness: the deepest cell holding the whole group (common key
-    // prefix of its first and last positions, as tree order is key order)
-    // bounds the box, and that cell's particle count bounds what the box
-    // can contain; a group whose cell holds more than kGroupCell Q particles
-    // (it straddles a jump of the Morton curve, an outlier, a dense core
-    // and its outskirts) walks per query instead.
-    const int t1 = t0 + kWave * Q - 1 < n ? t0 + kWave * Q - 1 : n - 1;
-    const int c = common_levels(tword, ext, t0, t1);
-    const int cell_count = run_end(tword, ext, n, t1, c) - run_start(tword, ext, t0, c) + 1;
-    bool per_query = cell_count > kGroupCell * Q;
-    WalkAcc<FEATURES> acc[Q];
-    if (!per_query) {
-        const uint64_t lt = lane == 0 ? 0ull : ((~0ull) >> (kWave - lane));
-        int ccnt = 0, lcnt = 0;   // staged cells, leaves (wave-uniform)
+    // the group: tree positions t0 + lane
+    const int t = t0 + lane;
+    const bool valid = t < n;
+    const float4 q = s4[valid ? t : n - 1];
+    const float qx = q.x, qy = q.y, qz = q.z;
+    const uint64_t lt = lane == 0 ? 0ull : ((~0ull) >> (kWave - lane));
+    WalkAcc<FEATURES> acc;
+    bool per_query = false;   // this lane's sub-group could not share a list
+
+    // One shared-list walk for the members [a, b] (tree positions) of the
+    // group: box acceptance against the members' bounding box. Returns false,
+    // with the members' sums cleared, when the pending cells would overflow
+    // the stack.
+    auto shared_walk = [&](int a, int b) -> bool {
+        const bool member = valid && t >= a && t <= b;
+        const float bx0 = wave_min(member ? qx : 3.0e38f);
+        const float by0 = wave_min(member ? qy : 3.0e38f);
+        const float bz0 = wave_min(member ? qz : 3.0e38f);
+        const float bx1 = wave_max(member ? qx : -3.0e38f);
+        const float by1 = wave_max(member ? qy : -3.0e38f);
+        const float bz1 = wave_max(member ? qz : -3.0e38f);
+        int ccnt = 0, lcnt = 0;            // staged cells, leaves (wave-uniform)
+        unsigned ncell = 0, nleaf = 0;     // evaluated per member (wave-uniform)
+        selfhit[lane] = 0;
         // evaluate the staged lists for every member
         auto flush = [&]() {
             wave_lds_fence();
@@ -1376,35 +1376,39 @@ __global__ __launch_bounds__(kGroupThreads) void group_walk_kernel(
                 lleaf[e] = s4[ltag[e]];
             }
             wave_lds_fence();
-            for (int e = 0; e < ccnt; ++e) {
-                const float4 p = lcell[e];
-                const float4 q0 = FEATURES ? lq[2 * e] : make_float4(0.f, 0.f, 0.f, 0.f);
-                const float4 q1 = FEATURES ? lq[2 * e + 1] : make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-                for (int k = 0; k < Q; ++k) {
-                    const float dx = p.x - qx[k];
-                    const float dy = p.y - qy[k];
-                    const float dz = p.z - qz[k];
+            if (member) {
+                for (int e = 0; e < ccnt; ++e) {
+                    const float4 p = lcell[e];
+                    const float dx = p.x - qx;
+                    const float dy = p.y - qy;
+                    const float dz = p.z - qz;
                     if (FEATURES) {
+                        const float4 q0 = lq[2 * e];
+                        const float4 q1 = lq[2 * e + 1];
                         // the plain group walk's softened distance, bit for bit
-                        acc[k].cell(dx, dy, dz, fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2))),
-                                    fmaf(dx, dx, fmaf(dy, dy, dz * dz)), p.w, q0, q1);
+                        acc.cell(dx, dy, dz, fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2))),
+                                 fmaf(dx, dx, fmaf(dy, dy, dz * dz)), p.w, q0, q1);
                     } else {
                         // softened distance straight from the FMA chain (the
                         // unsoftened one only feeds the features)
-                        acc[k].cell_softened(dx, dy, dz, p.w, eps2);
+                        acc.cell_softened(dx, dy, dz, p.w, eps2);
+                    }
+                }
+                if (eps2 > 0.f) {
+                    for (int e = 0; e < lcnt; ++e) {
+                        const float4 p = lleaf[e];
+                        acc.leaf_term(p.x - qx, p.y - qy, p.z - qz, p.w, eps2);
+                    }
+                } else {
+                    for (int e = 0; e < lcnt; ++e) {
+                        const float4 p = lleaf[e];
+                        acc.leaf_or_self_term(p.x - qx, p.y - qy, p.z - qz, p.w, ltag[e] == t,
+                                              eps2);
                     }
                 }
             }
-            for (int e = 0; e < lcnt; ++e) {
-                const float4 p = lleaf[e];
-                const int tag = ltag[e];
-#pragma unroll
-                for (int k = 0; k < Q; ++k) {
-                    acc[k].leaf_or_self(p.x - qx[k], p.y - qy[k], p.z - qz[k], p.w, tag == t[k],
-                                        eps2);
-                }
-            }
+            ncell += static_cast<unsigned>(ccnt);
+            nleaf += static_cast<unsigned>(lcnt);
             ccnt = 0;
             lcnt = 0;
             wave_lds_fence();
@@ -1416,6 +1420,7 @@ __global__ __launch_bounds__(kGroupThreads) void group_walk_kernel(
             }
             sp = 1;
         }
+        bool overflow = false;
         wave_lds_fence();
         while (sp > 0) {
             // pop at most 64 cells, and few enough that their children (8
@@ -1476,13 +1481,18 @@ __global__ __launch_bounds__(kGroupThreads) void group_walk_kernel(
                         flush();
                     }
                     if (ch < -1) {
-                        ltag[lcnt + static_cast<int>(__popcll(lb & lt))] = -ch - 2;
+                        const int tag = -ch - 2;
+                        ltag[lcnt + static_cast<int>(__popcll(lb & lt))] = tag;
+                        const unsigned rel = static_cast<unsigned>(tag - t0);
+                        if (rel < static_cast<unsigned>(kWave)) {
+                            selfhit[rel] = 1;
+                        }
                     }
                     lcnt += static_cast<int>(__popcll(lb));
                 }
                 const int np = static_cast<int>(__popcll(ib));
                 if (sp + np > kGroupStack) {
-                    per_query = true;
+                    overflow = true;
                 } else if (np != 0) {
                     if (ch >= 0) {
                         stack[sp + static_cast<int>(__popcll(ib & lt))] = ch;
@@ -1490,51 +1500,108 @@ __global__ __launch_bounds__(kGroupThreads) void group_walk_kernel(
                     sp += np;
                 }
             }
-            if (per_query) {
+            if (overflow) {
                 break;
             }
             wave_lds_fence();   // pushes visible before the next pops
         }
-        if (!per_query && ccnt + lcnt > 0) {
+        if (!overflow && ccnt + lcnt > 0) {
             flush();
         }
-        if (per_query) {
-            // stack overflow: discard the partial sums and walk per query
-#pragma unroll
-            for (int k = 0; k < Q; ++k) {
-                acc[k] = WalkAcc<FEATURES>();
+        wave_lds_fence();
+        if (member) {
+            if (overflow) {
+                acc = WalkAcc<FEATURES>();
+            } else {
+                // each member evaluated every staged entry but its own leaf
+                acc.approx = ncell;
+                acc.exact = nleaf - selfhit[lane];
             }
         }
+        wave_lds_fence();   // selfhit read before the next walk clears it
+        return !overflow;
+    };
+
+    // Sub-ranges of the group, split at the Morton curve's jumps until each
+    // is compact: its smallest enclosing cell holds <= kGroupCell particles
+    // (a range straddling a jump, an outlier, a dense core and its outskirts
+    // would otherwise get a box with most of the tree in its near field). A
+    // range that is not compact is split at the octant changes of its common
+    // cell's children (tree order is key order, so each octant is a run); a
+    // range that cannot be split within the budget walks per query.
+    const int last = t0 + kWave - 1 < n ? t0 + kWave - 1 : n - 1;
+    const int max_level = ext != nullptr ? kMaxLevels : 21;
+    int rsp = 1, walks = 0;
+    if (lane == 0) {
+        ranges[0] = make_int2(t0, last);
     }
-    if (per_query) {
-        int skip_until[Q];
-#pragma unroll
-        for (int k = 0; k < Q; ++k) {
-            skip_until[k] = t[k] < n ? 0 : 0x7FFFFFFF;
+    wave_lds_fence();
+    while (rsp > 0) {
+        --rsp;
+        const int2 r = ranges[rsp];
+        const int a = __builtin_amdgcn_readfirstlane(r.x);
+        const int b = __builtin_amdgcn_readfirstlane(r.y);
+        wave_lds_fence();   // the pop is read before a push reuses the slot
+        const int c = a < b ? common_levels(tword, ext, a, b) : max_level;
+        bool shared = c >= max_level;
+        if (!shared) {
+            const int count = run_end(tword, ext, n, b, c) - run_start(tword, ext, a, c) + 1;
+            shared = count <= kGroupCell;
         }
+        uint64_t cut = 0;
+        if (!shared && rsp + 8 <= kRangeStack && walks + rsp + 2 <= kMaxSubWalks) {
+            // octant of each member's depth c + 1 cell; runs of equal octants
+            // (a and b differ in that octant, so there are at least two)
+            const int dig = digit_at(tword, ext, valid ? t : a, c);
+            const int prev = __shfl_up(dig, 1);
+            cut = __ballot(t > a && t <= b && dig != prev);
+        }
+        if (cut != 0) {
+            int s = a;
+            while (cut != 0) {
+                const int j = __builtin_ctzll(cut);
+                cut &= cut - 1;
+                if (lane == 0) {
+                    ranges[rsp] = make_int2(s, t0 + j - 1);
+                }
+                ++rsp;
+                s = t0 + j;
+            }
+            if (lane == 0) {
+                ranges[rsp] = make_int2(s, b);
+            }
+            ++rsp;
+            wave_lds_fence();
+            continue;
+        }
+        if (shared) {
+            ++walks;
+            if (!shared_walk(a, b)) {
+                per_query = per_query || (valid && t >= a && t <= b);
+            }
+        } else {
+            per_query = per_query || (valid && t >= a && t <= b);
+        }
+    }
+    if (__ballot(per_query) != 0) {
+        // reference lists for the lanes whose range could not share a list
+        int skip_until = per_query ? 0 : 0x7FFFFFFF;
         int i = 0;
         while (i < nodes) {
             const int iu = __builtin_amdgcn_readfirstlane(i);
             const NodeF32 cur = n32[iu];
             const float4 q0 = FEATURES ? pquad[2 * iu] : make_float4(0.f, 0.f, 0.f, 0.f);
             const float4 q1 = FEATURES ? pquad[2 * iu + 1] : make_float4(0.f, 0.f, 0.f, 0.f);
-            bool descend = false;
-#pragma unroll
-            for (int k = 0; k < Q; ++k) {
-                descend |= walk_visit<FEATURES>(cur, i, t[k], qx[k], qy[k], qz[k], eps2, theta2,
-                                                skip_until[k], acc[k], q0, q1);
-            }
+            const bool descend = walk_visit<FEATURES>(cur, i, t, qx, qy, qz, eps2, theta2,
+                                                      skip_until, acc, q0, q1);
             i = cur.leaf >= 0 || __builtin_amdgcn_ballot_w64(descend) != 0 ? i + 1 : cur.skip;
         }
     }
     unsigned long long wa = 0, we = 0;
-#pragma unroll
-    for (int k = 0; k < Q; ++k) {
-        if (t[k] < n) {
-            wa += acc[k].approx;
-            we += acc[k].exact;
-            acc[k].store(t[k], tperm, G, ax, ay, az, features);
-        }
+    if (valid) {
+        wa = acc.approx;
+        we = acc.exact;
+        acc.store(t, tperm, G, ax, ay, az, features);
     }
     for (int off = 32; off > 0; off >>= 1) {
         wa += __shfl_xor(wa, off);
@@ -1879,12 +1946,12 @@ NBodySimProStatus bh_walk(Engine* e, const NBodySimProGravity* g, bool fp64_exac
             return (blocks_q + kXcds - 1) / kXcds * kXcds;
         };
         if (features != nullptr) {
-            group_walk_kernel<true, 1><<<grid_q(1), kGroupThreads, 0, st>>>(
+            group_walk_kernel<true><<<grid_q(1), kGroupThreads, 0, st>>>(
                 nodes32, nn, gn, ni_cells, B<float4>(s.s4), B<uint32_t>(s.tperm),
                 B<uint64_t>(s.tword), s.ext_tree, ni, Gf, e2, th2, e->ax, e->ay, e->az, cnts,
                 features, gq, gqp);
         } else {
-            group_walk_kernel<false, 1><<<grid_q(1), kGroupThreads, 0, st>>>(
+            group_walk_kernel<false><<<grid_q(1), kGroupThreads, 0, st>>>(
                 nodes32, nn, gn, ni_cells, B<float4>(s.s4), B<uint32_t>(s.tperm),
                 B<uint64_t>(s.tword), s.ext_tree, ni, Gf, e2, th2, e->ax, e->ay, e->az, cnts,
                 nullptr, nullptr, nullptr);
