@@ -65,7 +65,12 @@ enum {
      * consecutive particles shares one list accepted against the group's
      * bounding box (every particle's list then refines its reference list:
      * more interactions, same or smaller error). */
-    NBX_FLAG_BH_REFERENCE_LISTS = 1u
+    NBX_FLAG_BH_REFERENCE_LISTS = 1u,
+    /* group walk only: a group of 64 that is not compact (its smallest
+     * enclosing cell holds more than 2048 particles) walks each of its
+     * particles with its reference list instead of being split into compact
+     * parts that each share a list. Slower; for comparisons and tests. */
+    NBX_FLAG_BH_NO_SPLIT = 2u
 };
 
 typedef struct NBXConfig {

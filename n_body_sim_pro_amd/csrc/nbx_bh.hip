@@ -1319,7 +1319,7 @@ __global__ __launch_bounds__(kGroupThreads) void group_walk_kernel(
     double* __restrict__ ax, double* __restrict__ ay,
     double* __restrict__ az, unsigned long long* __restrict__ counters,
     float* __restrict__ features, const float4* __restrict__ gquad,
-    const float4* __restrict__ pquad) {
+    const float4* __restrict__ pquad, int max_walks) {
     constexpr int kWaves = kGroupThreads / kWave;
     // per wave: pending cells; staged cells (COM, mass; with FEATURES the
     // quadrupole record); staged leaves (tree position, then position and
@@ -1530,7 +1530,8 @@ __global__ __launch_bounds__(kGroupThreads) void group_walk_kernel(
     // would otherwise get a box with most of the tree in its near field). A
     // range that is not compact is split at the octant changes of its common
     // cell's children (tree order is key order, so each octant is a run); a
-    // range that cannot be split within the budget walks per query.
+    // range that cannot be split within the budget of `max_walks` shared
+    // walks per wave walks per query.
     const int last = t0 + kWave - 1 < n ? t0 + kWave - 1 : n - 1;
     const int max_level = ext != nullptr ? kMaxLevels : 21;
     int rsp = 1, walks = 0;
@@ -1551,7 +1552,7 @@ __global__ __launch_bounds__(kGroupThreads) void group_walk_kernel(
             shared = count <= kGroupCell;
         }
         uint64_t cut = 0;
-        if (!shared && rsp + 8 <= kRangeStack && walks + rsp + 2 <= kMaxSubWalks) {
+        if (!shared && rsp + 8 <= kRangeStack && walks + rsp + 2 <= max_walks) {
             // octant of each member's depth c + 1 cell; runs of equal octants
             // (a and b differ in that octant, so there are at least two)
             const int dig = digit_at(tword, ext, valid ? t : a, c);
@@ -1943,6 +1944,8 @@ NBodySimProStatus bh_walk(Engine* e, const NBodySimProGravity* g, bool fp64_exac
         const float e2 = static_cast<float>(g->softening_squared);
         const float th2 = static_cast<float>(theta2);
         unsigned long long* cnts = B<unsigned long long>(s.counters);
+        // one shared walk per group (no split) or the split's budget
+        const int max_walks = (e->flags & NBX_FLAG_BH_NO_SPLIT) ? 1 : kMaxSubWalks;
         auto grid_q = [&](int q) {
             const unsigned blocks_q = grid_for(s.n, kGroupThreads * q);
             return (blocks_q + kXcds - 1) / kXcds * kXcds;
@@ -1951,12 +1954,12 @@ NBodySimProStatus bh_walk(Engine* e, const NBodySimProGravity* g, bool fp64_exac
             group_walk_kernel<true><<<grid_q(1), kGroupThreads, 0, st>>>(
                 nodes32, nn, gn, ni_cells, B<float4>(s.s4), B<uint32_t>(s.tperm),
                 B<uint64_t>(s.tword), s.ext_tree, ni, Gf, e2, th2, e->ax, e->ay, e->az, cnts,
-                features, gq, gqp);
+                features, gq, gqp, max_walks);
         } else {
             group_walk_kernel<false><<<grid_q(1), kGroupThreads, 0, st>>>(
                 nodes32, nn, gn, ni_cells, B<float4>(s.s4), B<uint32_t>(s.tperm),
                 B<uint64_t>(s.tword), s.ext_tree, ni, Gf, e2, th2, e->ax, e->ay, e->az, cnts,
-                nullptr, nullptr, nullptr);
+                nullptr, nullptr, nullptr, max_walks);
         }
     } else {
         const unsigned wblocks = grid_for(s.n, kThreads * kWalkQueries);

@@ -230,3 +230,26 @@ def test_bh_empty_system_is_invalid(nbx):
         e.compute_force("hip_barnes_hut", eps=0.01)
     assert ex.value.status == 1
     e.close()
+
+
+def test_group_walk_fallback_is_the_reference_list_walk(nbx):
+    """With the split off (NBX_FLAG_BH_NO_SPLIT) a group that is not compact
+    walks each particle with its reference list: those particles get exactly
+    the per-particle walk's accelerations, every other particle exactly the
+    default group walk's (its group is compact, one shared walk either way)."""
+    n = 200_000
+    d = nbx.two_galaxy(n, seed=42)
+    acc = {}
+    for key, kw in (("default", {}), ("no_split", {"split": False}),
+                    ("reference", {"reference_lists": True})):
+        with nbx.Engine(n, theta=0.7, **kw) as e:
+            e.upload(d["pos"], d["m"])
+            e.compute_force("hip_barnes_hut", eps=0.01)
+            acc[key] = e.accelerations()
+    same_group = np.all(acc["no_split"] == acc["default"], axis=1)
+    same_ref = np.all(acc["no_split"] == acc["reference"], axis=1)
+    assert np.all(same_group | same_ref)
+    frac = 1.0 - same_group.mean()
+    print(f"particles on the per-particle fallback: {frac:.3%}")
+    assert 0.02 < frac < 0.3          # the non-compact groups (11.6% at 1M)
+    assert not np.any(same_ref & ~same_group & np.all(acc["reference"] == acc["default"], axis=1))
