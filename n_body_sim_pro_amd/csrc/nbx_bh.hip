@@ -1109,7 +1109,9 @@ struct WalkAcc {
             qaz = fmaf(fmaf(c, dz, -qdz), inv5, qaz);
             const float q2 = d2 * q1.w;   // (distance / cell edge)^2
             q2_sum += q2;
-            q2_min = fminf(q2_min, q2);
+            // (d / edge)^2 >= 0, where unsigned order is float order: one
+            // v_min_u32 and no NaN canonicalisation
+            q2_min = __uint_as_float(min(__float_as_uint(q2_min), __float_as_uint(q2)));
         }
     }
 
@@ -1325,10 +1327,17 @@ __global__ __launch_bounds__(kGroupThreads) void group_walk_kernel(
     // quadrupole record); staged leaves (tree position, then position and
     // mass); whether each lane's own leaf was staged; pending sub-ranges
     __shared__ int stack_all[kWaves][kGroupStack];
-    // FEATURES stages fewer cells per flush: their quadrupole records take LDS
+    // FEATURES stages fewer cells per flush: their quadrupole records take LDS.
+    // (Measured on MI355X, learned walk at 10M: 96 or 128 staged cells, 8.9 or
+    // 10.4 KB of LDS, are 4% and 17% slower; six waves per SIMD through 80
+    // VGPRs and 64 staged leaves is 5% slower. A list holds up to one step's
+    // accepted cells, one per lane, so it cannot be shorter than a wave.)
+    // A staged cell is kCellWords float4 in a row (COM and mass, then with
+    // FEATURES the two quadrupole float4), so one LDS pointer walks the list.
     constexpr int kList = FEATURES ? kGroupList / 2 : kGroupList;
-    __shared__ float4 lcell_all[kWaves][kList];
-    __shared__ float4 lq_all[FEATURES ? kWaves : 1][FEATURES ? 2 * kList : 1];
+    static_assert(kList >= kWave, "a step stages up to one accepted cell per lane");
+    constexpr int kCellWords = FEATURES ? 3 : 1;
+    __shared__ float4 lcell_all[kWaves][kList * kCellWords];
     __shared__ int ltag_all[kWaves][kGroupLeaves];
     __shared__ float4 lleaf_all[kWaves][kGroupLeaves];
     __shared__ unsigned char selfhit_all[kWaves][kWave];
@@ -1337,7 +1346,6 @@ __global__ __launch_bounds__(kGroupThreads) void group_walk_kernel(
     const int wid = threadIdx.x / kWave;
     int* stack = stack_all[wid];
     float4* lcell = lcell_all[wid];
-    float4* lq = lq_all[FEATURES ? wid : 0];
     int* ltag = ltag_all[wid];
     float4* lleaf = lleaf_all[wid];
     unsigned char* selfhit = selfhit_all[wid];
@@ -1380,16 +1388,17 @@ __global__ __launch_bounds__(kGroupThreads) void group_walk_kernel(
             wave_lds_fence();
             if (member) {
                 for (int e = 0; e < ccnt; ++e) {
-                    const float4 p = lcell[e];
+                    const float4 p = lcell[kCellWords * e];
                     const float dx = p.x - qx;
                     const float dy = p.y - qy;
                     const float dz = p.z - qz;
                     if (FEATURES) {
-                        const float4 q0 = lq[2 * e];
-                        const float4 q1 = lq[2 * e + 1];
-                        // the plain group walk's softened distance, bit for bit
-                        acc.cell(dx, dy, dz, fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2))),
-                                 fmaf(dx, dx, fmaf(dy, dy, dz * dz)), p.w, q0, q1);
+                        const float4 q0 = lcell[kCellWords * e + 1];
+                        const float4 q1 = lcell[kCellWords * e + 2];
+                        // softened distance from the unsoftened one (within an
+                        // ulp of the plain walk's FMA chain)
+                        const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                        acc.cell(dx, dy, dz, d2 + eps2, d2, p.w, q0, q1);
                     } else {
                         // softened distance straight from the FMA chain (the
                         // unsoftened one only feeds the features)
@@ -1462,10 +1471,10 @@ __global__ __launch_bounds__(kGroupThreads) void group_walk_kernel(
             const uint64_t cbal = __ballot(accepted);
             if (accepted) {
                 const int pos = ccnt + static_cast<int>(__popcll(cbal & lt));
-                lcell[pos] = c0;
+                lcell[kCellWords * pos] = c0;
                 if (FEATURES) {
-                    lq[2 * pos] = gq0;
-                    lq[2 * pos + 1] = gq1;
+                    lcell[kCellWords * pos + 1] = gq0;
+                    lcell[kCellWords * pos + 2] = gq1;
                 }
             }
             ccnt += static_cast<int>(__popcll(cbal));
