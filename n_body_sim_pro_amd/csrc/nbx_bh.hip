@@ -1336,6 +1336,12 @@ __global__ __launch_bounds__(kGroupThreads) void group_walk_kernel(
     // FEATURES the two quadrupole float4), so one LDS pointer walks the list.
     constexpr int kList = FEATURES ? kGroupList / 2 : kGroupList;
     static_assert(kList >= kWave, "a step stages up to one accepted cell per lane");
+    // The plain walk evaluates its staged lists four entries per loop trip
+    // (LDS loads run ahead, one address update per four; same summation
+    // order). Measured on MI355X against one per trip: walk 7% faster at 1M
+    // particles and 5% at 10M, at 78 VGPRs; eight per trip takes 98 VGPRs.
+    // The feature walk stays at one (two per trip: 99 VGPRs, fewer waves).
+    constexpr int kEvalUnroll = FEATURES ? 1 : 4;
     constexpr int kCellWords = FEATURES ? 3 : 1;
     __shared__ float4 lcell_all[kWaves][kList * kCellWords];
     __shared__ int ltag_all[kWaves][kGroupLeaves];
@@ -1387,6 +1393,7 @@ __global__ __launch_bounds__(kGroupThreads) void group_walk_kernel(
             }
             wave_lds_fence();
             if (member) {
+#pragma unroll kEvalUnroll
                 for (int e = 0; e < ccnt; ++e) {
                     const float4 p = lcell[kCellWords * e];
                     const float dx = p.x - qx;
@@ -1406,6 +1413,7 @@ __global__ __launch_bounds__(kGroupThreads) void group_walk_kernel(
                     }
                 }
                 if (eps2 > 0.f) {
+#pragma unroll kEvalUnroll
                     for (int e = 0; e < lcnt; ++e) {
                         const float4 p = lleaf[e];
                         acc.leaf_term(p.x - qx, p.y - qy, p.z - qz, p.w, eps2);
