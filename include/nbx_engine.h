@@ -84,8 +84,8 @@ typedef struct NBXConfig {
 typedef struct NBXTimings {
     double force_ms;      /* whole force evaluation */
     double bbox_keys_ms;  /* BH: bounding cube + keys */
-    double sort_ms;       /* BH: radix sort(s) */
-    double tree_ms;       /* BH: node construction */
+    double sort_ms;       /* BH: radix sort into Morton order */
+    double tree_ms;       /* BH: tree order, depths and node construction */
     double com_ms;        /* BH: centre-of-mass pass */
     double walk_ms;       /* BH: walk (the dominant BH kernel) */
     double kernel_ms;     /* dominant kernel of the last evaluation */

@@ -1733,6 +1733,7 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error,
     NBX_TRY(radix_sort_pairs(st, B<uint64_t>(s.sk0), B<uint32_t>(s.sv0), B<uint64_t>(s.sk1),
                              B<uint32_t>(s.sv1), n, 63, s.hist, error,
                              B<uint64_t>(s.mkey_sorted), B<uint32_t>(s.mperm)));
+    NBX_HIP(error, hipEventRecord(e->ev[9], st));
 
     // Tree order, neighbour depths and per-depth node counts for a given
     // ascending true-path order (null: Morton order), then one host read of
@@ -1826,7 +1827,6 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error,
         if (hflags[kDCount + d] != 0) deepest = d;
     }
     NBX_TRY(exclusive_scan_u32(st, B<uint32_t>(s.cnt), B<uint32_t>(s.scan), n, s.scan_part, error));
-    NBX_HIP(error, hipEventRecord(e->ev[2], st));
 
     const size_t nodes = n + internal;
     NBX_TRY(s.icom.need<double4>(internal + 1, error));
@@ -2009,14 +2009,14 @@ NBodySimProStatus barnes_hut(Engine* e, const NBodySimProGravity* g, bool fp64_e
                              NBodySimProError* error) {
     NBX_TRY(bh_build(e, !fp64_exact, error));
     NBX_TRY(bh_walk(e, g, fp64_exact, nullptr, error));
-    float t[7] = {0, 0, 0, 0, 0, 0, 0};
-    for (int k = 1; k <= 6; ++k) {
+    float t[10] = {0};
+    for (int k : {1, 3, 4, 5, 6, 9}) {
         NBX_HIP(error, hipEventElapsedTime(&t[k], e->ev[0], e->ev[k]));
     }
     NBXTimings tm{};
     tm.bbox_keys_ms = t[1];
-    tm.sort_ms = t[2] - t[1];
-    tm.tree_ms = t[3] - t[2];
+    tm.sort_ms = t[9] - t[1];
+    tm.tree_ms = t[3] - t[9];
     tm.com_ms = t[4] - t[3];
     tm.walk_ms = t[6] - t[5];
     tm.force_ms = t[6];

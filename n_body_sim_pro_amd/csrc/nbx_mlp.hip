@@ -729,16 +729,16 @@ NBodySimProStatus bh_learned(Engine* e, const NBodySimProGravity* g, NBodySimPro
     NBX_HIP(error, hipGetLastError());
     NBX_HIP(error, hipEventRecord(e->ev[8], e->stream));
     NBX_HIP(error, hipEventSynchronize(e->ev[8]));
-    float t[9] = {0};
-    for (int k : {1, 2, 3, 4, 6, 8}) {
+    float t[10] = {0};
+    for (int k : {1, 3, 4, 6, 8, 9}) {
         NBX_HIP(error, hipEventElapsedTime(&t[k], e->ev[0], e->ev[k]));
     }
     float mlp_ms = 0.f;
     NBX_HIP(error, hipEventElapsedTime(&mlp_ms, e->ev[7], e->ev[8]));
     NBXTimings tm{};
     tm.bbox_keys_ms = t[1];
-    tm.sort_ms = t[2] - t[1];
-    tm.tree_ms = t[3] - t[2];
+    tm.sort_ms = t[9] - t[1];
+    tm.tree_ms = t[3] - t[9];
     tm.com_ms = t[4] - t[3];
     float walk_ms = 0.f;
     NBX_HIP(error, hipEventElapsedTime(&walk_ms, e->ev[5], e->ev[6]));
