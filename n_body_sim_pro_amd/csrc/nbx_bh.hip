@@ -1757,7 +1757,12 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error,
                                       hipMemcpyDeviceToHost, st));
         NBX_HIP(error, hipMemcpyAsync(s.root, root, 4 * sizeof(double), hipMemcpyDeviceToHost,
                                       st));
-        NBX_HIP(error, hipStreamSynchronize(st));
+        NBX_HIP(error, hipEventRecord(e->ev[10], st));
+        // node numbering (internal nodes started per tree position, scanned)
+        // runs on the device while the host waits for the flags
+        NBX_TRY(exclusive_scan_u32(st, B<uint32_t>(s.cnt), B<uint32_t>(s.scan), n, s.scan_part,
+                                   error));
+        NBX_HIP(error, hipEventSynchronize(e->ev[10]));
         return N_BODY_SIM_PRO_STATUS_OK;
     };
     // The Morton order is the tree order unless some true-path key differs
@@ -1826,7 +1831,6 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error,
         internal += hflags[kDCount + d];
         if (hflags[kDCount + d] != 0) deepest = d;
     }
-    NBX_TRY(exclusive_scan_u32(st, B<uint32_t>(s.cnt), B<uint32_t>(s.scan), n, s.scan_part, error));
 
     const size_t nodes = n + internal;
     NBX_TRY(s.icom.need<double4>(internal + 1, error));
