@@ -153,6 +153,30 @@ def test_direct_fp32_subsample_at_65536(nbx, oracle):
     assert np.median(err) <= 1e-6
 
 
+@pytest.mark.parametrize("n", [1, 2, 3, 63, 64, 65, 255, 256, 257, 300, 1023, 1025, 2049])
+def test_direct_fp32_ragged_sizes(nbx, oracle, n):
+    """Sizes around the wave (64), the compensation tile (256 sources) and the
+    split of each target's sources into four ranges of whole tiles (1025:
+    ranges of 2, 2, 1 and 0 tiles), with and without softening and with
+    G != 1 (G multiplies each target's sum once)."""
+    rng = np.random.default_rng(1000 + n)
+    pos = rng.normal(size=(n, 3))
+    p = Particles(n, {"px": pos[:, 0], "py": pos[:, 1], "pz": pos[:, 2],
+                      "m": rng.uniform(0.5, 1.5, n)})
+    for G, eps in ((1.0, 0.0), (1.0, 0.01), (2.5, 0.01)):
+        acc = gpu_force(nbx, p, "hip_direct_fp32", eps, G=G)
+        q = p.copy()
+        oracle.nbo_direct_fp64(C.byref(q.view), C.byref(gravity(G, eps)))
+        ref = q.acc()
+        assert np.all(np.isfinite(acc))
+        if n == 1:
+            assert np.all(acc == 0.0)
+            continue
+        err = rel_err_per_particle(acc, ref)
+        assert np.median(err) <= 1e-6, (G, eps, np.median(err))
+        assert err.max() < 1e-4, (G, eps, err.max())
+
+
 def test_direct_fp32_zero_softening_is_finite(nbx):
     d = nbx.random_cloud(1000, seed=3)
     p = from_gen(d)
