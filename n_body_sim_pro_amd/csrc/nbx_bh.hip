@@ -185,39 +185,37 @@ __device__ inline int run_start(const uint64_t* w, const uint64_t* ext, int t, i
     return lo;
 }
 
-// ---------------------------------------------------------------- inputs
+// ---------------------------------------------------------------- inputs and bounding cube
 // Build inputs as one (x, y, z, m) record per particle: FP32-rounded and
 // widened back to FP64 (hip_barnes_hut) or the FP64 values (the FP64 walk).
 template <bool ROUND>
-__global__ void build_inputs_kernel(const double* __restrict__ px, const double* __restrict__ py,
-                                    const double* __restrict__ pz, const double* __restrict__ m,
-                                    double4* __restrict__ b4, int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {
-        if (ROUND) {
-            b4[i] = make_double4(static_cast<double>(static_cast<float>(px[i])),
-                                 static_cast<double>(static_cast<float>(py[i])),
-                                 static_cast<double>(static_cast<float>(pz[i])),
-                                 static_cast<double>(static_cast<float>(m[i])));
-        } else {
-            b4[i] = make_double4(px[i], py[i], pz[i], m[i]);
-        }
+__device__ inline double4 build_input(const double* __restrict__ px, const double* __restrict__ py,
+                                      const double* __restrict__ pz, const double* __restrict__ m,
+                                      int i) {
+    if (ROUND) {
+        return make_double4(static_cast<double>(static_cast<float>(px[i])),
+                            static_cast<double>(static_cast<float>(py[i])),
+                            static_cast<double>(static_cast<float>(pz[i])),
+                            static_cast<double>(static_cast<float>(m[i])));
     }
+    return make_double4(px[i], py[i], pz[i], m[i]);
 }
 
-// ---------------------------------------------------------------- bounding cube
 // barnes_hut.c:453-483. min/max are exact, so the reduction order is free.
+// The same grid-stride pass writes the build inputs.
 constexpr int kBBoxBlocks = 512;
 
-__global__ __launch_bounds__(kThreads) void bbox_partial_kernel(const double4* __restrict__ b4,
-                                                               int n,
-                                                               double* __restrict__ partial) {
+template <bool ROUND>
+__global__ __launch_bounds__(kThreads) void bbox_partial_kernel(
+    const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
+    const double* __restrict__ m, double4* __restrict__ b4, int n, double* __restrict__ partial) {
     __shared__ double s[6][kThreads];
-    const double4 p0 = b4[0];
+    const double4 p0 = build_input<ROUND>(px, py, pz, m, 0);
     double lo[3] = {p0.x, p0.y, p0.z};
     double hi[3] = {lo[0], lo[1], lo[2]};
     for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
-        const double4 q = b4[i];
+        const double4 q = build_input<ROUND>(px, py, pz, m, i);
+        b4[i] = q;
         const double p[3] = {q.x, q.y, q.z};
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
@@ -1686,11 +1684,6 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error,
     NBX_HIP(error, hipEventRecord(e->ev[0], st));
     NBX_TRY(s.b4.need<double4>(n, error));
     double4* b4 = B<double4>(s.b4);
-    if (fp32_inputs) {
-        build_inputs_kernel<true><<<g, kThreads, 0, st>>>(e->px, e->py, e->pz, e->m, b4, ni);
-    } else {
-        build_inputs_kernel<false><<<g, kThreads, 0, st>>>(e->px, e->py, e->pz, e->m, b4, ni);
-    }
     const unsigned bb_blocks = g < static_cast<unsigned>(kBBoxBlocks) ? g : kBBoxBlocks;
     NBX_TRY(s.partial.need<double>(6 * kBBoxBlocks + 8, error));
     NBX_TRY(s.flags.need<unsigned>(kFlagWords, error));
@@ -1720,7 +1713,13 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error,
     double* root = B<double>(s.partial) + 6 * kBBoxBlocks;
     unsigned* flags = B<unsigned>(s.flags);
     NBX_HIP(error, hipMemsetAsync(flags, 0, kFlagWords * sizeof(unsigned), st));
-    bbox_partial_kernel<<<bb_blocks, kThreads, 0, st>>>(b4, ni, B<double>(s.partial));
+    if (fp32_inputs) {
+        bbox_partial_kernel<true><<<bb_blocks, kThreads, 0, st>>>(e->px, e->py, e->pz, e->m, b4,
+                                                                  ni, B<double>(s.partial));
+    } else {
+        bbox_partial_kernel<false><<<bb_blocks, kThreads, 0, st>>>(e->px, e->py, e->pz, e->m, b4,
+                                                                   ni, B<double>(s.partial));
+    }
     bbox_final_kernel<<<1, kThreads, 0, st>>>(B<double>(s.partial), static_cast<int>(bb_blocks),
                                               root);
     keys_kernel<<<g, kThreads, 0, st>>>(b4, ni, root, B<uint64_t>(s.mkey),
