@@ -1879,7 +1879,9 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error,
         run += static_cast<int>(hflags[kDCount + d]);
     }
     // Deepest level first. Runs of small levels share one single-workgroup
-    // launch; a large level gets a full-grid launch of its own.
+    // launch; a large level gets a full-grid launch of its own. (Measured on
+    // MI355X, 1M two-galaxy: small levels up to 512 cells 0.113 ms for the
+    // whole pass, up to 1,536 cells 0.119 ms, up to 2,560 cells 0.136 ms.)
     constexpr unsigned kSmallLevel = 512;
     for (int d = deepest; d >= 0;) {
         const unsigned count = hflags[kDCount + d];
