@@ -253,3 +253,21 @@ def test_group_walk_fallback_is_the_reference_list_walk(nbx):
     print(f"particles on the per-particle fallback: {frac:.3%}")
     assert 0.02 < frac < 0.3          # the non-compact groups (11.6% at 1M)
     assert not np.any(same_ref & ~same_group & np.all(acc["reference"] == acc["default"], axis=1))
+
+
+def test_bh_phase_timings_are_consistent(nbx):
+    """The per-phase HIP-event timings of a Barnes-Hut evaluation: every phase
+    is positive, and bounding cube/keys, sort, tree construction, centre of
+    mass and walk fit inside the whole evaluation (their brackets do not
+    overlap)."""
+    n = 100_000
+    d = nbx.two_galaxy(n, seed=5)
+    with nbx.Engine(n, theta=0.7) as e:
+        e.upload(d["pos"], d["m"])
+        for alg in ("hip_barnes_hut", "hip_barnes_hut_fp64", "hip_bh_learned"):
+            e.compute_force(alg, eps=0.01)
+            t = e.timings()
+            phases = ("bbox_keys_ms", "sort_ms", "tree_ms", "com_ms", "walk_ms")
+            assert all(t[k] > 0.0 for k in phases), (alg, t)
+            assert sum(t[k] for k in phases) <= t["force_ms"] * 1.001 + 1e-3, (alg, t)
+            assert t["kernel_ms"] == t["walk_ms"]
