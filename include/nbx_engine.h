@@ -89,9 +89,10 @@ typedef struct NBXTimings {
     double com_ms;        /* BH: centre-of-mass pass */
     double walk_ms;       /* BH: walk (the dominant BH kernel) */
     double kernel_ms;     /* dominant kernel of the last evaluation */
-    double integrate_ms;  /* last advance(): total */
+    double integrate_ms;  /* last advance(): total; 0 after a force evaluation alone */
     double kernel_bytes;  /* algorithmic bytes moved by the dominant kernel */
     double kernel_flops;  /* algorithmic flops of the dominant kernel (20 per interaction) */
+    double mlp_ms;        /* hip_bh_learned: the learned correction kernel; 0 otherwise */
 } NBXTimings;
 
 /* Name <-> algorithm for the `--algorithm` vocabulary; -1 / NULL when unknown. */

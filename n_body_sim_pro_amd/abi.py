@@ -66,4 +66,5 @@ class Timings(C.Structure):
     """NBXTimings in include/nbx_engine.h (milliseconds from HIP events)."""
     _fields_ = [(n, C.c_double) for n in (
         "force_ms", "bbox_keys_ms", "sort_ms", "tree_ms", "com_ms", "walk_ms", "kernel_ms",
-        "integrate_ms")] + [("kernel_bytes", C.c_double), ("kernel_flops", C.c_double)]
+        "integrate_ms")] + [("kernel_bytes", C.c_double), ("kernel_flops", C.c_double),
+                            ("mlp_ms", C.c_double)]

@@ -1758,11 +1758,13 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error,
                                       st));
         NBX_HIP(error, hipEventRecord(e->ev[10], st));
         // node numbering (internal nodes started per tree position, scanned)
-        // runs on the device while the host waits for the flags
-        NBX_TRY(exclusive_scan_u32(st, B<uint32_t>(s.cnt), B<uint32_t>(s.scan), n, s.scan_part,
-                                   error));
+        // runs on the device while the host waits for the flags; the copies
+        // into hflags and s.root land before this returns, on success or not
+        const NBodySimProStatus scanned =
+            exclusive_scan_u32(st, B<uint32_t>(s.cnt), B<uint32_t>(s.scan), n, s.scan_part,
+                               error);
         NBX_HIP(error, hipEventSynchronize(e->ev[10]));
-        return N_BODY_SIM_PRO_STATUS_OK;
+        return scanned;
     };
     // The Morton order is the tree order unless some true-path key differs
     // from its Morton key (keys_kernel raises the flag): build on it and redo

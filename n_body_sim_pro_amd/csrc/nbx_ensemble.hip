@@ -71,7 +71,9 @@ __device__ inline void kb_add(float& s, float& c, float v) {
     c += e;
 }
 
-template <int R>
+// ZERO_EPS: unsoftened sums; the self pair (r2 == 0) and coincident pairs
+// contribute nothing instead of rsq(0) = inf times 0 = NaN.
+template <int R, bool ZERO_EPS>
 __device__ inline void tile_force(const float4* __restrict__ pos, double* __restrict__ ax,
                                   double* __restrict__ ay, double* __restrict__ az,
                                   const WorkItem& w, int lane, float G, float eps2, int tile) {
@@ -102,7 +104,10 @@ __device__ inline void tile_force(const float4* __restrict__ pos, double* __rest
                 const float dy = q.y - yi[r];
                 const float dz = q.z - zi[r];
                 const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
-                const float inv = __builtin_amdgcn_rsqf(r2);
+                float inv = __builtin_amdgcn_rsqf(r2);
+                if (ZERO_EPS) {
+                    inv = r2 > 0.f ? inv : 0.f;
+                }
                 const float s = (q.w * inv) * (inv * inv);
                 px[r] = fmaf(s, dx, px[r]);
                 py[r] = fmaf(s, dy, py[r]);
@@ -128,6 +133,7 @@ __device__ inline void tile_force(const float4* __restrict__ pos, double* __rest
 }
 
 // Persistent launch: waves pull items from their XCD's queue until it is empty.
+template <bool ZERO_EPS>
 __global__ __launch_bounds__(kThreads) void ens_force_kernel(
     const float4* __restrict__ pos, double* __restrict__ ax, double* __restrict__ ay,
     double* __restrict__ az, const WorkItem* __restrict__ items, int nitems,
@@ -147,11 +153,11 @@ __global__ __launch_bounds__(kThreads) void ens_force_kernel(
         }
         const WorkItem w = items[item];
         if (w.r == 4) {
-            tile_force<4>(pos, ax, ay, az, w, lane, G, eps2, tile);
+            tile_force<4, ZERO_EPS>(pos, ax, ay, az, w, lane, G, eps2, tile);
         } else if (w.r == 2) {
-            tile_force<2>(pos, ax, ay, az, w, lane, G, eps2, tile);
+            tile_force<2, ZERO_EPS>(pos, ax, ay, az, w, lane, G, eps2, tile);
         } else {
-            tile_force<1>(pos, ax, ay, az, w, lane, G, eps2, tile);
+            tile_force<1, ZERO_EPS>(pos, ax, ay, az, w, lane, G, eps2, tile);
         }
         item = __builtin_amdgcn_readfirstlane(static_cast<int>(kn)) * kQueues + qid;
     }
@@ -334,9 +340,15 @@ NBodySimProStatus upload_items(Ensemble& s, const std::vector<WorkItem>& v,
 void launch_force(Ensemble& s, float G, float eps2) {
     ens_reset_kernel<<<1, kQueues, 0, s.stream>>>(s.queue);
     const unsigned grid = static_cast<unsigned>(s.cu_count * s.tile.blocks_per_cu);
-    ens_force_kernel<<<grid, kThreads, 0, s.stream>>>(s.pos4, s.ax, s.ay, s.az,
-                                                      s.items.as<WorkItem>(), s.nitems, s.queue,
-                                                      G, eps2, s.tile.source_tile);
+    if (eps2 > 0.f) {
+        ens_force_kernel<false><<<grid, kThreads, 0, s.stream>>>(
+            s.pos4, s.ax, s.ay, s.az, s.items.as<WorkItem>(), s.nitems, s.queue, G, eps2,
+            s.tile.source_tile);
+    } else {
+        ens_force_kernel<true><<<grid, kThreads, 0, s.stream>>>(
+            s.pos4, s.ax, s.ay, s.az, s.items.as<WorkItem>(), s.nitems, s.queue, G, eps2,
+            s.tile.source_tile);
+    }
 }
 
 void drop_graph(Ensemble& s) {
@@ -593,10 +605,22 @@ NBodySimProStatus nbx_ensemble_advance(NBXEnsemble* e, const NBodySimProGravity*
         nbx::drop_graph(s);
         NBX_HIP(error, hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
         nbx::enqueue_steps(s, G, eps2, dt, graph_steps);
+        // End the capture whatever the launches returned, so the stream never
+        // stays in capture mode; a failed capture leaves no graph behind.
+        const hipError_t launched = hipGetLastError();
         hipGraph_t graph = nullptr;
-        NBX_HIP(error, hipStreamEndCapture(s.stream, &graph));
+        const hipError_t ended = hipStreamEndCapture(s.stream, &graph);
         s.graph = graph;
-        NBX_HIP(error, hipGraphInstantiate(&s.graph_exec, s.graph, nullptr, nullptr, 0));
+        if (launched != hipSuccess || ended != hipSuccess) {
+            nbx::drop_graph(s);
+            NBX_HIP(error, launched != hipSuccess ? launched : ended);
+        }
+        const hipError_t inst = hipGraphInstantiate(&s.graph_exec, s.graph, nullptr, nullptr, 0);
+        if (inst != hipSuccess) {
+            s.graph_exec = nullptr;
+            nbx::drop_graph(s);
+            NBX_HIP(error, inst);
+        }
         s.graph_steps = graph_steps;
         s.graph_dt = dt;
         s.graph_g = g->gravitational_constant;

@@ -745,7 +745,7 @@ NBodySimProStatus bh_learned(Engine* e, const NBodySimProGravity* g, NBodySimPro
     tm.walk_ms = walk_ms;
     tm.force_ms = t[8];
     tm.kernel_ms = walk_ms;
-    tm.integrate_ms = mlp_ms;   // reported as the correction's own time
+    tm.mlp_ms = mlp_ms;
     const double interactions = static_cast<double>(e->stats.accepted_approximations +
                                                     e->stats.exact_interactions);
     tm.kernel_flops = 20.0 * interactions;

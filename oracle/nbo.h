@@ -16,8 +16,10 @@
  *                            long-double brute force on tiny N.
  *   nbo_direct_openmp ...... bit-identical to nbo_direct_fp64.
  *   nbo_bh_* ............... keys/permutation/accelerations/stats bit-identical
- *                            to oracle/_ref; published interaction counts
- *                            39,416 / 32,591 / 14,355 (400-particle cloud);
+ *                            to oracle/_ref (400-particle cloud, seed 7:
+ *                            154,231 / 107,700 / 37,920 interactions at
+ *                            theta 0.1 / 0.3 / 0.7); the reference tests'
+ *                            two-body and theta trade-off assertions;
  *                            theta -> 0 reduces to a direct sum over leaves;
  *                            stable_sort((key, index)) for the permutation.
  *   nbo_leapfrog_advance ... bit-identical to oracle/_ref; two-body circular

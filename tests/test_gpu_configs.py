@@ -1,0 +1,332 @@
+"""The five benchmark configurations at their stated sizes, against the oracle.
+
+First the checker itself: nbx_engine_direct_fp64_targets, the FP64 direct sum
+behind every error figure at N >= 1M, is bit-identical to the oracle's
+OpenMP direct sum over the same targets (unsorted and repeated indices).
+Then, each at the configuration's own size:
+  config 2  two-galaxy N=1,000,000, FP32 positions and masses: 63-bit keys
+            (and their 30-bit view), sorted order, root cube and topology
+            bit-exact against the oracle's build, before and after 3 KDK steps
+            with dt=1e-3;
+  config 3  Plummer N=262,144: FP32 all-pairs against the oracle on a seeded
+            4096-particle sample (median <= 1e-6) and 100-step energy drift
+            against the FP64 all-pairs drift;
+  config 4  4096 Plummer clusters, sizes log-uniform in [64, 4096]: forces of a
+            seeded set of clusters (the largest included) against the oracle,
+            and the energy drift of a 50-step graph-captured leapfrog against
+            the oracle's FP64 leapfrog on 16 clusters;
+  config 5  two-galaxy N=10,000,000: the learned theta=1.0 path against plain
+            theta=0.5 on a seeded 8192-particle sample, and against the
+            quadrupole-corrected walk without the network.
+"""
+import ctypes as C
+import time
+
+import numpy as np
+import pytest
+
+from nbx_testlib import Diagnostics, Particles, gravity, rel_err_per_particle
+from test_gpu_bh import from_gen, sort_rows
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def oracle_subset(oracle, d, idx, G=1.0, eps=0.01):
+    """FP64 direct sum of the oracle (reference scalar order) for particles idx."""
+    p = from_gen(d)
+    g = gravity(G, eps)
+    t = np.ascontiguousarray(idx, dtype=np.int64)
+    out = np.zeros((len(t), 3))
+    st = oracle.nbo_direct_openmp_subset(C.byref(p.view), C.byref(g),
+                                         t.ctypes.data_as(C.POINTER(C.c_int64)), len(t),
+                                         out.ctypes.data_as(C.POINTER(C.c_double)))
+    assert st == 0
+    return out
+
+
+def oracle_build(oracle, pos, m):
+    """The oracle's tree on (pos, m): root, keys, permutation, node rows."""
+    n = len(m)
+    p = Particles(n, {"px": pos[:, 0], "py": pos[:, 1], "pz": pos[:, 2], "m": m})
+    t = oracle.nbo_bh_create()
+    try:
+        assert oracle.nbo_bh_build(t, C.byref(p.view)) == 0
+        keys = np.zeros(n, dtype=np.uint64)
+        perm = np.zeros(n, dtype=np.int64)
+        root = np.zeros(4)
+        oracle.nbo_bh_keys(t, keys.ctypes.data_as(C.POINTER(C.c_uint64)),
+                           perm.ctypes.data_as(C.POINTER(C.c_int64)))
+        oracle.nbo_bh_root(t, root.ctypes.data_as(C.POINTER(C.c_double)))
+        rows = np.zeros((oracle.nbo_bh_node_count(t), 11))
+        oracle.nbo_bh_export_nodes(t, rows.ctypes.data_as(C.POINTER(C.c_double)))
+    finally:
+        oracle.nbo_bh_destroy(t)
+    return root, keys, perm, rows
+
+
+def fp32_rounded(a):
+    return np.asarray(a, dtype=np.float64).astype(np.float32).astype(np.float64)
+
+
+# ---- the checker -------------------------------------------------------------
+
+def test_direct_fp64_targets_bit_identical_to_oracle_at_1m(nbx, oracle):
+    """4096 targets of the 1M two-galaxy input, unsorted, with repeats and the
+    first and last particle: the GPU checker and the oracle agree bit for bit."""
+    n = 1_000_000
+    d = nbx.two_galaxy(n, seed=42)
+    rng = np.random.default_rng(11)
+    idx = rng.choice(n, 4090, replace=False)
+    idx = np.concatenate([idx, idx[:4], [0, n - 1]])
+    rng.shuffle(idx)
+    _say(f"checker: two-galaxy N={n}, {idx.size} targets")
+    with nbx.Engine(n) as e:
+        e.upload(d["pos"], d["m"])
+        t0 = time.perf_counter()
+        gpu = e.direct_fp64_targets(idx, 1.0, 0.01)
+        t_gpu = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ref = oracle_subset(oracle, d, idx)
+    _say(f"checker: gpu {t_gpu:.2f} s, oracle {time.perf_counter() - t0:.2f} s")
+    assert np.array_equal(gpu, ref)
+
+
+def test_direct_fp64_targets_equal_full_rows(nbx):
+    """The subset kernel returns exactly the rows of the full FP64 all-pairs
+    kernel (unsorted and repeated targets, G != 1, eps = 0)."""
+    n = 20_011
+    d = nbx.random_cloud(n, seed=4)
+    idx = np.random.default_rng(2).integers(0, n, 777)
+    with nbx.Engine(n) as e:
+        e.upload(d["pos"], d["m"])
+        for G, eps in ((1.0, 0.01), (0.37, 0.0)):
+            e.compute_force("hip_direct_fp64", G, eps)
+            full = e.accelerations()
+            assert np.array_equal(e.direct_fp64_targets(idx, G, eps), full[idx])
+
+
+# ---- config 2 ----------------------------------------------------------------
+
+def _check_tree(e, oracle, pos, m):
+    root, keys, perm = e.bh_morton()
+    rows = e.bh_export_nodes()
+    oroot, okeys, operm, orows = oracle_build(oracle, fp32_rounded(pos), fp32_rounded(m))
+    assert np.array_equal(root, oroot)
+    assert np.array_equal(keys, okeys)
+    assert np.array_equal(keys >> np.uint64(33), okeys >> np.uint64(33))
+    assert np.array_equal(perm, operm)
+    assert rows.shape == orows.shape
+    # depth, centre, half, count, mass (FP64 from the FP32-rounded inputs)
+    assert np.array_equal(sort_rows(rows)[:, :7], sort_rows(orows)[:, :7])
+
+
+def test_config2_1m_keys_order_topology_over_three_steps(nbx, oracle):
+    """Config 2 at N=1,000,000, theta=0.7, eps=0.01: the FP32 walk's tree
+    matches the oracle's build on the same FP32-rounded inputs, for the
+    initial state and after 3 leapfrog steps (dt=1e-3) through the FP32 walk."""
+    n = 1_000_000
+    d = nbx.two_galaxy(n, seed=42)
+    _say(f"config 2: N={n} initial build")
+    with nbx.Engine(n, theta=0.7) as e:
+        e.upload(d["pos"], d["m"], d["vel"])
+        e.compute_force("hip_barnes_hut", eps=0.01)
+        _check_tree(e, oracle, d["pos"], d["m"])
+        _say("config 2: 3 KDK steps")
+        e.advance("hip_barnes_hut", dt=1e-3, steps=3, eps=0.01)
+        s = e.download()
+        assert not np.array_equal(s["pos"], d["pos"])
+        _check_tree(e, oracle, s["pos"], d["m"])
+
+
+def test_group_walk_unsoftened_refines_reference_lists(nbx):
+    """eps = 0 (the reference's default softening): the group walk's leaf terms
+    take the self-masked branch. It makes at least the reference lists'
+    interactions and is at least as accurate against the FP64 direct sum."""
+    n = 200_000
+    d = nbx.two_galaxy(n, seed=8)
+    idx = np.sort(np.random.default_rng(4).choice(n, 2048, replace=False))
+    out = {}
+    for ref_lists in (True, False):
+        with nbx.Engine(n, theta=0.7, reference_lists=ref_lists) as e:
+            e.upload(d["pos"], d["m"])
+            e.compute_force("hip_barnes_hut", eps=0.0)
+            s = e.stats()
+            acc = e.accelerations()
+            ref = e.direct_fp64_targets(idx, 1.0, 0.0)
+        assert np.all(np.isfinite(acc))
+        err = np.linalg.norm(acc[idx] - ref, axis=1) / np.linalg.norm(ref, axis=1)
+        out[ref_lists] = (s["accepted_approximations"] + s["exact_interactions"], err)
+        _say(f"eps=0 reference_lists={ref_lists}: interactions {out[ref_lists][0]} "
+             f"median err {np.median(err):.3e} max {err.max():.3e}")
+    assert out[False][0] >= out[True][0]
+    assert np.median(out[False][1]) <= np.median(out[True][1])
+    assert np.median(out[True][1]) < 5e-2
+
+
+# ---- config 3 ----------------------------------------------------------------
+
+def test_config3_262144_error_against_oracle_and_drift(nbx, oracle):
+    """Config 3: N=262,144 Plummer, eps=0.01. Median relative error of the FP32
+    all-pairs accelerations on a seeded 4096-particle sample against the
+    oracle <= 1e-6 (max <= 1e-5); over 100 leapfrog steps (dt=1e-3) the FP32
+    path's relative energy drift stays within 1e-8 of the FP64 path's."""
+    n = 262_144
+    d = nbx.plummer(n, seed=42)
+    idx = np.sort(np.random.default_rng(7).choice(n, 4096, replace=False))
+    _say("config 3: FP32 all-pairs N=262144")
+    t0 = time.perf_counter()
+    ref = oracle_subset(oracle, d, idx)
+    _say(f"config 3: oracle sample {time.perf_counter() - t0:.1f} s")
+    drift = {}
+    with nbx.Engine(n) as e:
+        e.upload(d["pos"], d["m"], d["vel"])
+        e.compute_force("hip_direct_fp32", eps=0.01)
+        err = rel_err_per_particle(e.accelerations()[idx], ref)
+        _say(f"config 3: median {np.median(err):.3e} max {err.max():.3e}")
+        assert np.median(err) <= 1e-6
+        assert err.max() <= 1e-5
+        for alg in ("hip_direct_fp32", "hip_direct_fp64"):
+            e.upload(d["pos"], d["m"], d["vel"])
+            e.compute_force(alg, eps=0.01)
+            q0 = e.diagnostics(eps=0.01)
+            e.advance(alg, dt=1e-3, steps=100, eps=0.01)
+            q1 = e.diagnostics(eps=0.01)
+            e0 = q0["kinetic_energy"] + q0["potential_energy"]
+            e1 = q1["kinetic_energy"] + q1["potential_energy"]
+            drift[alg] = (e1 - e0) / abs(e0)
+            _say(f"config 3: {alg} 100-step drift {drift[alg]:.3e}")
+    assert abs(drift["hip_direct_fp32"]) < 1e-6
+    assert abs(drift["hip_direct_fp32"] - drift["hip_direct_fp64"]) < 1e-8
+
+
+# ---- config 4 ----------------------------------------------------------------
+
+def _cluster(d, off, n):
+    sl = slice(off, off + n)
+    return Particles(n, {"px": d["pos"][sl, 0], "py": d["pos"][sl, 1], "pz": d["pos"][sl, 2],
+                         "vx": d["vel"][sl, 0], "vy": d["vel"][sl, 1], "vz": d["vel"][sl, 2],
+                         "m": d["m"][sl]})
+
+
+def test_config4_4096_clusters_forces_and_drift(nbx, oracle):
+    """Config 4: 4096 Plummer clusters, sizes log-uniform in [64, 4096] (seed
+    42), eps=0.01, autotuned tiles. Forces of 24 seeded clusters plus the
+    largest and the smallest against the oracle (median <= 1e-6); per-cluster
+    energy drift over a 50-step graph-captured leapfrog (dt=1e-3) against the
+    oracle's FP64 leapfrog on 16 of them (largest included)."""
+    sizes = nbx.ensemble_sizes(4096, lo=64, hi=4096, seed=42)
+    d = nbx.ensemble_clusters(sizes, seed=42)
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    assert sizes.min() >= 64 and sizes.max() <= 4096 and len(sizes) == 4096
+    rng = np.random.default_rng(13)
+    pick = list(rng.choice(len(sizes), 24, replace=False))
+    pick += [int(np.argmax(sizes)), int(np.argmin(sizes))]
+    _say(f"config 4: {int(sizes.sum())} particles, largest cluster {int(sizes.max())}")
+    with nbx.Ensemble(sizes) as ens:
+        ens.upload(d["pos"], d["vel"], d["m"])
+        ens.force(eps=0.01)
+        ens.autotune(eps=0.01)
+        ens.force(eps=0.01)
+        acc = ens.download()["acc"]
+        e0 = ens.energies(eps=0.01).sum(1)
+        ens.advance(1e-3, 50, graph_steps=50, eps=0.01)
+        e1 = ens.energies(eps=0.01).sum(1)
+    g = gravity(1.0, 0.01)
+    errs = []
+    for k in pick:
+        p = _cluster(d, offs[k], int(sizes[k]))
+        oracle.nbo_direct_openmp(C.byref(p.view), C.byref(g))
+        errs.append(rel_err_per_particle(acc[offs[k]:offs[k] + sizes[k]], p.acc()))
+    err = np.concatenate(errs)
+    _say(f"config 4: forces median {np.median(err):.3e} max {err.max():.3e}")
+    assert np.median(err) <= 1e-6
+    assert err.max() < 1e-4
+    fn = C.cast(oracle.nbo_direct_openmp, C.c_void_p)
+    t0 = time.perf_counter()
+    for k in pick[:15] + [int(np.argmax(sizes))]:
+        p = _cluster(d, offs[k], int(sizes[k]))
+        oracle.nbo_direct_openmp(C.byref(p.view), C.byref(g))
+        for _ in range(50):
+            assert oracle.nbo_leapfrog_advance(C.byref(p.view), C.byref(g), 1e-3, fn) == 0
+        q = Diagnostics()
+        oracle.nbo_diagnostics_global(C.byref(p.view), C.byref(q))
+        u = C.c_double()
+        oracle.nbo_potential_energy(C.byref(p.view), C.byref(g), C.byref(u))
+        eo = q.kinetic_energy + u.value
+        drift_gpu = (e1[k] - e0[k]) / abs(e0[k])
+        drift_cpu = (eo - e0[k]) / abs(e0[k])
+        assert abs(drift_gpu - drift_cpu) < 1e-6, (k, int(sizes[k]), drift_gpu, drift_cpu)
+    _say(f"config 4: oracle leapfrog {time.perf_counter() - t0:.1f} s")
+
+
+def test_ensemble_unsoftened_is_finite_and_matches_oracle(nbx, oracle):
+    """eps = 0: each target's own position is among its sources; the self pair
+    must contribute nothing (not NaN)."""
+    sizes = nbx.ensemble_sizes(40, lo=64, hi=900, seed=3)
+    d = nbx.ensemble_clusters(sizes, seed=3)
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    with nbx.Ensemble(sizes) as ens:
+        ens.upload(d["pos"], d["vel"], d["m"])
+        ens.force(eps=0.0)
+        acc = ens.download()["acc"]
+    assert np.all(np.isfinite(acc))
+    g = gravity(1.0, 0.0)
+    errs = []
+    for k in range(len(sizes)):
+        p = _cluster(d, offs[k], int(sizes[k]))
+        oracle.nbo_direct_openmp(C.byref(p.view), C.byref(g))
+        errs.append(rel_err_per_particle(acc[offs[k]:offs[k] + sizes[k]], p.acc()))
+    err = np.concatenate(errs)
+    _say(f"ensemble eps=0: median {np.median(err):.3e} max {err.max():.3e}")
+    assert np.median(err) <= 1e-6
+    assert err.max() < 1e-3
+
+
+# ---- config 5 ----------------------------------------------------------------
+
+def test_config5_10m_learned_against_theta05_and_quadrupole(nbx):
+    """Config 5 evaluation: N=10,000,000 two-galaxy collision, eps=0.01, error
+    by the FP64 direct sum (the pinned checker) on a seeded 8192-particle
+    sample. The learned theta=1.0 path beats plain theta=0.5's median error,
+    and beats the same theta=1.0 walk corrected by its quadrupole force alone
+    (no network)."""
+    from n_body_sim_pro_amd import learned as L
+    n = 10_000_000
+    d = nbx.two_galaxy(n, seed=42)
+    idx = np.sort(np.random.default_rng(7).choice(n, 8192, replace=False))
+    _say(f"config 5: N={n}")
+    mlp = L.Mlp.load(L.WEIGHTS)
+    try:
+        with nbx.Engine(n, theta=0.5) as e:
+            e.upload(d["pos"], d["m"])
+            ref = e.direct_fp64_targets(idx, 1.0, 0.01)
+            e.compute_force("hip_barnes_hut", 1.0, 0.01)
+            half = e.accelerations()[idx]
+            e.theta = 1.0
+            feats, frames = e.bh_features(1.0, 0.01, frames=True)
+            base = e.accelerations()[idx]
+            e.set_mlp(mlp)
+            e.compute_force("hip_bh_learned", 1.0, 0.01)
+            corr = e.accelerations()[idx]
+            e.set_mlp(None)
+            e.compute_force("hip_barnes_hut", 1.0, 0.01)
+            plain1 = e.accelerations()[idx]
+    finally:
+        mlp.close()
+    f = feats[idx].astype(np.float64)
+    fr = frames[idx].astype(np.float64)
+    a_n = np.linalg.norm(base, axis=1)
+    q = a_n[:, None] * (f[:, 5:6] * fr[:, 0] + f[:, 6:7] * fr[:, 1])
+    quad = base + q
+    r = np.linalg.norm(ref, axis=1)
+    med = {k: float(np.median(np.linalg.norm(v - ref, axis=1) / r))
+           for k, v in (("theta0.5", half), ("theta1.0", plain1), ("quadrupole", quad),
+                        ("learned", corr))}
+    _say("config 5 median errors: " + ", ".join(f"{k} {v:.4e}" for k, v in med.items()))
+    assert med["learned"] < med["theta0.5"]
+    assert med["learned"] < med["quadrupole"]
+    assert med["learned"] < 0.25 * med["theta1.0"]
