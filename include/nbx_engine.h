@@ -177,6 +177,26 @@ NBodySimProStatus nbx_hip_bh_learned(const NBodySimProParticleSystemView* view,
                                      const NBodySimProGravity* gravity, void* context,
                                      NBodySimProError* error);
 
+/*
+ * FP32 all-pairs accelerations that allocate nothing. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ *
+ * Same arithmetic and kernels as nbx_hip_direct_fp32, hence identical
+ * accelerations, but every pointer is a DEVICE pointer on the current HIP
+ * device (count doubles each), and the work is only enqueued on `stream`
+ * (a hipStream_t; NULL = the null stream): the caller synchronizes. The
+ * FP32 staging lives in a fixed per-device array of the library, so
+ * count <= 1048576 (larger counts: INVALID_ARGUMENT), and calls that overlap
+ * on one GPU race on it.
+ */
+NBodySimProStatus nbx_direct_fp32_noalloc(const double* positions_x, const double* positions_y,
+                                          const double* positions_z, const double* masses,
+                                          double* accelerations_x, double* accelerations_y,
+                                          double* accelerations_z, size_t count,
+                                          const NBodySimProGravity* gravity, void* stream,
+                                          NBodySimProError* error);
+
 /* Number of visible HIP devices (0 when none). */
 int nbx_device_count(void);
 

@@ -19,6 +19,8 @@
 //   are combined in LDS in range order. G multiplies each target's sum once,
 //   in FP64, so a pair costs 12 full-rate VALU instructions and one
 //   v_rsq_f32.
+#include <mutex>
+
 #include "nbx_internal.h"
 #include "nbx_learned.h"
 
@@ -297,5 +299,81 @@ extern "C" NBodySimProStatus nbx_engine_direct_fp64_targets(NBXEngine* engine,
     NBX_HIP(error, hipMemcpyAsync(out, dout, 3 * count * sizeof(double), hipMemcpyDeviceToHost,
                                   e->stream));
     NBX_HIP(error, hipStreamSynchronize(e->stream));
+    return N_BODY_SIM_PRO_STATUS_OK;
+}
+
+// ---- FP32 all-pairs without allocation (nbx_direct_fp32_noalloc) ----
+//
+// The FP32 (x, y, z, m) staging that nbx_hip_direct_fp32 keeps in its engine
+// lives here in a fixed device array instead: each GPU the library runs on
+// holds its own copy (16 MiB) until the process ends. Every call on a GPU
+// shares that GPU's copy, so calls that overlap on one GPU (on different
+// streams, or from different host threads) race. The staging is fully
+// rewritten for [0, n) by each call, so nothing needs clearing.
+namespace nbx {
+namespace {
+constexpr size_t kNoAllocMaxParticles = size_t(1) << 20;
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_noalloc_mutex;
+void* g_noalloc_staging[kNoAllocMaxDevices] = {};
+}  // namespace
+}  // namespace nbx
+
+static __device__ __align__(256) float4 scratch[nbx::kNoAllocMaxParticles];
+
+extern "C" NBodySimProStatus nbx_direct_fp32_noalloc(
+    const double* positions_x, const double* positions_y, const double* positions_z,
+    const double* masses, double* accelerations_x, double* accelerations_y,
+    double* accelerations_z, size_t count, const NBodySimProGravity* gravity, void* stream,
+    NBodySimProError* error) {
+    if (gravity == nullptr) {
+        NBX_FAIL(error, N_BODY_SIM_PRO_STATUS_INVALID_ARGUMENT, "gravity is required");
+    }
+    if (count == 0) {
+        return N_BODY_SIM_PRO_STATUS_OK;
+    }
+    if (positions_x == nullptr || positions_y == nullptr || positions_z == nullptr ||
+        masses == nullptr || accelerations_x == nullptr || accelerations_y == nullptr ||
+        accelerations_z == nullptr) {
+        NBX_FAIL(error, N_BODY_SIM_PRO_STATUS_INVALID_ARGUMENT,
+                 "positions, masses and accelerations must be device pointers, not null");
+    }
+    if (count > nbx::kNoAllocMaxParticles) {
+        NBX_FAIL(error, N_BODY_SIM_PRO_STATUS_INVALID_ARGUMENT,
+                 "nbx_direct_fp32_noalloc takes at most 1048576 particles");
+    }
+    int device = -1;
+    NBX_HIP(error, hipGetDevice(&device));
+    if (device < 0 || device >= nbx::kNoAllocMaxDevices) {
+        NBX_FAIL(error, N_BODY_SIM_PRO_STATUS_UNSUPPORTED_PLATFORM, "device ordinal above 63");
+    }
+    float4* staging = nullptr;
+    {
+        // first call on this device (current): the device address of its copy
+        std::lock_guard<std::mutex> lock(nbx::g_noalloc_mutex);
+        if (nbx::g_noalloc_staging[device] == nullptr) {
+            void* p = nullptr;
+            NBX_HIP(error, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+            nbx::g_noalloc_staging[device] = p;
+        }
+        staging = static_cast<float4*>(nbx::g_noalloc_staging[device]);
+    }
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const int n = static_cast<int>(count);
+    nbx::refresh_pos4_kernel<<<nbx::grid_for(count, 256), 256, 0, st>>>(
+        positions_x, positions_y, positions_z, masses, staging, n);
+    // same kernel, source split and tile as nbx_hip_direct_fp32 (identical results)
+    constexpr int S = 4;
+    const unsigned grid = nbx::grid_for(count, 256 / S);
+    const double G = gravity->gravitational_constant;
+    const float eps2 = static_cast<float>(gravity->softening_squared);
+    if (eps2 > 0.f) {
+        nbx::direct_fp32_kernel<S, false><<<grid, 256, 0, st>>>(
+            staging, accelerations_x, accelerations_y, accelerations_z, n, G, eps2, 256);
+    } else {
+        nbx::direct_fp32_kernel<S, true><<<grid, 256, 0, st>>>(
+            staging, accelerations_x, accelerations_y, accelerations_z, n, G, eps2, 256);
+    }
+    NBX_HIP(error, hipGetLastError());
     return N_BODY_SIM_PRO_STATUS_OK;
 }
