@@ -488,6 +488,11 @@ def learned_bh(nbx, cpu, n, seed, eps, evals, sample=8192):
     return out
 
 
+def progress(msg, t0=[time.perf_counter()]):
+    """One line per finished block on stderr (stdout carries only the JSON line)."""
+    print(f"bench: {msg} ({time.perf_counter() - t0[0]:.1f} s)", file=sys.stderr, flush=True)
+
+
 def measured_traffic(name):
     """HBM bytes per launch of a kernel from the committed rocprofv3 PMC
     summary (profiles/<name>_traffic.json), if one was recorded for this
@@ -615,20 +620,25 @@ def main():
     if tr is not None:
         out["roofline"]["traffic"] = tr.get("bytes_per_launch")
         out["roofline"]["traffic_source"] = tr.get("source")
+    progress("Barnes-Hut headline")
     cpu = None if args.no_cpu_baseline else CpuBaseline()
     if cpu is not None:
         out["cpu_environment"] = cpu.environment()
     if not args.no_direct_fp64:
         out["direct_fp64"] = direct_fp64(nbx, cpu, args.dp_n, args.eps, args.seed, args.dt)
+        progress("FP64 all-pairs block")
     if not args.no_all_pairs:
         out["all_pairs"] = all_pairs(nbx, cpu, args.ap_n, args.ap_evals, args.eps, args.seed,
                                      args.ap_drift_steps, args.dt)
+        progress("FP32 all-pairs block")
     if not args.no_ensemble:
         out["ensemble"] = ensemble(nbx, cpu, args.ens_clusters, args.ens_steps, args.eps,
                                    args.dt, args.seed)
+        progress("ensemble block")
     if not args.no_learned:
         out["learned"] = learned_bh(nbx, cpu, args.learned_n, args.seed, args.eps,
                                     args.learned_evals)
+        progress("learned block")
     if cpu is not None:
         # the reference's Barnes-Hut paths on the same FP32-rounded input
         view, cols = host_view(d["pos"].astype(np.float32).astype(np.float64),

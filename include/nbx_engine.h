@@ -70,7 +70,12 @@ enum {
      * enclosing cell holds more than 2048 particles) walks each of its
      * particles with its reference list instead of being split into compact
      * parts that each share a list. Slower; for comparisons and tests. */
-    NBX_FLAG_BH_NO_SPLIT = 2u
+    NBX_FLAG_BH_NO_SPLIT = 2u,
+    /* group walk only: a cell that the group's bounding-box test opens is
+     * still accepted when every member's own reference test accepts it
+     * (fewer interactions; each list still refines its members' reference
+     * lists). */
+    NBX_FLAG_BH_MEMBER_TEST = 4u
 };
 
 typedef struct NBXConfig {

@@ -2,16 +2,20 @@
  * nbx_learned.h - learned Barnes-Hut residual correction (C ABI, libnbx.so).
  *
  * A 3-layer MLP (16 -> 64 -> 64 -> 3, SiLU after the first two layers) maps
- * per-particle walk features to the residual (a_direct - a_BH) / |q|, q the
- * walk's quadrupole force (below; the residual of a theta = 1 walk lies
- * almost along q, so the target is a bounded, N-independent ratio),
- * expressed in the particle's local frame: e1 along a_BH, e2 along the
+ * per-particle walk features to the part of the walk's error that its
+ * quadrupole force q does not remove: (a_direct - a_BH - q) / |q| (q below;
+ * the residual of a theta = 1 walk lies almost along q, so q is added as is
+ * and the network predicts the bounded, N-independent remainder), expressed
+ * in the particle's local frame: e1 along a_BH, e2 along the
  * quadrupole force of the accepted cells perpendicular to e1 (or the
  * coordinate axis least aligned with e1 when that part vanishes), e3 = e1 x e2. Weights and
  * activations are BF16 on the MFMA units with FP32 accumulation; master
  * weights, gradients and Adam moments are FP32. The reference has no learned
  * component; it only counts accepted cells during its walk
  * (src/c/barnes_hut/barnes_hut.c:611), which is one of the features.
+ *
+ * The learned path's walk is the group walk with the member test of
+ * NBX_FLAG_BH_MEMBER_TEST always on.
  *
  * Features (raw, before standardisation; all frame-free), per particle, with
  * q = sum over accepted cells of the quadrupole force 5/2 (d.Q.d) d / r^7 -
@@ -25,14 +29,17 @@
  *   5  q along e1 / |a|                      6  |q perpendicular to e1| / |a|
  *   7  q along e1 / |q|                      8  |q perpendicular to e1| / |q|
  *   9  log2(|q| / |a| + 2^-40)               10-15 zero
- * The counts (3, 4) grow with N and are left out of the committed model.
+ * The counts (3, 4) grow with N and are left out of the committed model,
+ * as is the density (2).
  * A feature whose std is +inf standardises to 0, i.e. the model ignores it.
  * Parameter layout (float, nbx_mlp_param_count() values):
  *   W1[64][16], b1[64], W2[64][64], b2[64], W3[16][64] (rows 3..15 unused,
  *   kept zero), b3[16] (entries 3..15 unused).
  * The model standardises features as (f - mean) * float32(1 / std); its output y is
- * scaled: residual / |q| = (y0 e1 + y1 e2 + y2 e3) / out_scale (no accepted
- * cell: q = 0, no correction).
+ * scaled: (a_direct - a_BH - q) / |q| = (y0 e1 + y1 e2 + y2 e3) / out_scale,
+ * so the corrected acceleration is a_BH + q + |q| (y0 e1 + y1 e2 + y2 e3) /
+ * out_scale (no accepted cell: q = 0, no correction; a zero model gives the
+ * quadrupole-corrected walk a_BH + q).
  */
 #ifndef NBX_LEARNED_H
 #define NBX_LEARNED_H
@@ -75,10 +82,12 @@ NBodySimProStatus nbx_mlp_train(NBXMlp* mlp, int epochs, int batch, float learni
 /* Attach a trained model to an engine (NULL detaches): hip_bh_learned then
  * corrects its theta walk with it; without a model it is plain hip_barnes_hut. */
 NBodySimProStatus nbx_engine_set_mlp(NBXEngine* engine, NBXMlp* mlp, NBodySimProError* error);
-/* Build + walk on the device-resident state, returning the raw features
- * (n x 16, original particle order) and, when `frames` is not NULL, the local
- * frames (n x 9: e1, e2, e3); the walk's accelerations are left in the engine
- * as by hip_barnes_hut. */
+/* Build + walk (the learned path's walk) on the device-resident state,
+ * returning the raw features (n x 16, original particle order) and, when
+ * `frames` is not NULL, the local frames (n x 9: e1, e2, e3); the walk's
+ * accelerations are left in the engine (as by hip_barnes_hut with
+ * NBX_FLAG_BH_MEMBER_TEST; with NBX_FLAG_BH_REFERENCE_LISTS, its per-particle
+ * walk). */
 NBodySimProStatus nbx_engine_bh_features(NBXEngine* engine, const NBodySimProGravity* gravity,
                                          float* features, float* frames, NBodySimProError* error);
 /* FP64 direct sum (the reference's scalar order) for selected targets only:

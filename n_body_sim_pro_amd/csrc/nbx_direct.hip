@@ -28,53 +28,82 @@ namespace nbx {
 namespace {
 
 constexpr int kExactTargets = 64;
-constexpr int kExactSources = 32;
-constexpr int kExactThreads = 256;
+constexpr int kExactSources = 16;
+
+// One tile of terms: kExactSources sources x the block's 64 targets, each of
+// the block's WAVES waves forming kExactSources / WAVES sources' terms with
+// the reference's operations.
+template <int WAVES>
+__device__ inline void exact_terms(double (*terms)[kExactSources][kExactTargets],
+                                   const double* __restrict__ px, const double* __restrict__ py,
+                                   const double* __restrict__ pz, const double* __restrict__ m,
+                                   int n, int j0, int wave, int lane, double xi, double yi,
+                                   double zi, double G, double eps2) {
+    constexpr int kPerWave = kExactSources / WAVES;
+    static_assert(kExactSources % WAVES == 0, "whole sources per wave");
+#pragma unroll
+    for (int k = 0; k < kPerWave; ++k) {
+        const int jj = wave * kPerWave + k;
+        const int j = j0 + jj;
+        const int jc = j < n ? j : n - 1;
+        const double dx = px[jc] - xi;
+        const double dy = py[jc] - yi;
+        const double dz = pz[jc] - zi;
+        const double r2 = dx * dx + dy * dy + dz * dz + eps2;
+        const double inv = 1.0 / sqrt(r2);
+        const double inv3 = inv * inv * inv;
+        const double s = G * m[jc] * inv3;
+        terms[0][jj][lane] = s * dx;
+        terms[1][jj][lane] = s * dy;
+        terms[2][jj][lane] = s * dz;
+    }
+}
 
 // SUBSET: the block's targets are targets[k] for k = blockIdx.x * 64 + lane < count,
 // and the accelerations go to out[3k .. 3k+2].
-template <bool SUBSET>
-__global__ __launch_bounds__(kExactThreads) void direct_fp64_kernel(
+//
+// WAVES waves form the terms of a 16-source tile while waves 0-2 add the
+// previous tile's terms of one component each, in source order; two term
+// buffers alternate, one barrier per tile. (The serial sums are one add per
+// term; the terms, an IEEE divide and square root each, are spread over the
+// block's waves, which also hides their FP64 latency.) Sixteen waves suit
+// grids of under two blocks per CU, eight the larger ones; measured on
+// MI355X, ms per evaluation at N = 16384 / 65536 / 262144:
+//   16 waves: 1.18 / 11.96 / 179.9;  8 waves: 1.35 / 11.93 / 156.8;
+//   the earlier single-buffered 4-wave kernel: 1.69 / - / 192.
+template <bool SUBSET, int WAVES>
+__global__ __launch_bounds__(WAVES * 64) void direct_fp64_kernel(
     const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
     const double* __restrict__ m, double* __restrict__ ax, double* __restrict__ ay,
     double* __restrict__ az, int n, double G, double eps2, const int64_t* __restrict__ targets,
     int count, double* __restrict__ out) {
-    __shared__ double terms[3][kExactSources][kExactTargets];
+    __shared__ double terms[2][3][kExactSources][kExactTargets];
     const int lane = threadIdx.x & (kWave - 1);
-    const int wave = threadIdx.x / kWave;
+    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
     const int k = blockIdx.x * kExactTargets + lane;
     const bool valid = SUBSET ? k < count : k < n;
     const int i = !valid ? -1 : (SUBSET ? static_cast<int>(targets[k]) : k);
     const int ic = valid ? i : n - 1;
     const double xi = px[ic], yi = py[ic], zi = pz[ic];
     double acc = 0.0;
-    for (int j0 = 0; j0 < n; j0 += kExactSources) {
-        // Phase A: each wave forms 8 sources' terms for the block's 64 targets.
-#pragma unroll 2
-        for (int k = 0; k < kExactSources / 4; ++k) {
-            const int jj = wave * (kExactSources / 4) + k;
-            const int j = j0 + jj;
-            const int jc = j < n ? j : n - 1;
-            const double dx = px[jc] - xi;
-            const double dy = py[jc] - yi;
-            const double dz = pz[jc] - zi;
-            const double r2 = dx * dx + dy * dy + dz * dz + eps2;
-            const double inv = 1.0 / sqrt(r2);
-            const double inv3 = inv * inv * inv;
-            const double s = G * m[jc] * inv3;
-            terms[0][jj][lane] = s * dx;
-            terms[1][jj][lane] = s * dy;
-            terms[2][jj][lane] = s * dz;
-        }
-        __syncthreads();
-        // Phase B: in-order sums, one component per wave.
+    const int tiles = (n + kExactSources - 1) / kExactSources;
+    exact_terms<WAVES>(terms[0], px, py, pz, m, n, 0, wave, lane, xi, yi, zi, G, eps2);
+    __syncthreads();
+    for (int t = 0; t < tiles; ++t) {
+        const int j0 = t * kExactSources;
         if (wave < 3) {
+            // in-order sum of tile t, one component per wave
             const int jn = (n - j0) < kExactSources ? (n - j0) : kExactSources;
+            const double (*row)[kExactTargets] = terms[t & 1][wave];
             for (int jj = 0; jj < jn; ++jj) {
                 if (j0 + jj != i) {
-                    acc += terms[wave][jj][lane];
+                    acc += row[jj][lane];
                 }
             }
+        }
+        if (t + 1 < tiles) {
+            exact_terms<WAVES>(terms[(t + 1) & 1], px, py, pz, m, n, j0 + kExactSources, wave,
+                               lane, xi, yi, zi, G, eps2);
         }
         __syncthreads();
     }
@@ -85,6 +114,23 @@ __global__ __launch_bounds__(kExactThreads) void direct_fp64_kernel(
             double* o = wave == 0 ? ax : (wave == 1 ? ay : az);
             o[i] = acc;
         }
+    }
+}
+
+// Launches the FP64 all-pairs kernel over `count` targets (all n particles,
+// or the SUBSET `targets`) with the wave count that suits the grid.
+template <bool SUBSET>
+void launch_direct_fp64(hipStream_t st, int cu_count, const double* px, const double* py,
+                        const double* pz, const double* m, double* ax, double* ay, double* az,
+                        int n, double G, double eps2, const int64_t* targets, int count,
+                        double* out) {
+    const unsigned blocks = grid_for(static_cast<size_t>(count), kExactTargets);
+    if (blocks < 2u * static_cast<unsigned>(cu_count)) {
+        direct_fp64_kernel<SUBSET, 16><<<blocks, 16 * 64, 0, st>>>(px, py, pz, m, ax, ay, az, n,
+                                                                   G, eps2, targets, count, out);
+    } else {
+        direct_fp64_kernel<SUBSET, 8><<<blocks, 8 * 64, 0, st>>>(px, py, pz, m, ax, ay, az, n, G,
+                                                                 eps2, targets, count, out);
     }
 }
 
@@ -206,9 +252,9 @@ NBodySimProStatus direct_fp64(Engine* e, const NBodySimProGravity* g, NBodySimPr
         return N_BODY_SIM_PRO_STATUS_OK;
     }
     NBX_HIP(error, hipEventRecord(e->ev[0], e->stream));
-    direct_fp64_kernel<false><<<grid_for(n, kExactTargets), kExactThreads, 0, e->stream>>>(
-        e->px, e->py, e->pz, e->m, e->ax, e->ay, e->az, static_cast<int>(n),
-        g->gravitational_constant, g->softening_squared, nullptr, 0, nullptr);
+    launch_direct_fp64<false>(e->stream, e->cu_count, e->px, e->py, e->pz, e->m, e->ax, e->ay,
+                              e->az, static_cast<int>(n), g->gravitational_constant,
+                              g->softening_squared, nullptr, static_cast<int>(n), nullptr);
     NBX_HIP(error, hipGetLastError());
     NBX_HIP(error, hipEventRecord(e->ev[1], e->stream));
     NBX_HIP(error, hipEventSynchronize(e->ev[1]));
@@ -290,11 +336,10 @@ extern "C" NBodySimProStatus nbx_engine_direct_fp64_targets(NBXEngine* engine,
     double* dout = reinterpret_cast<double*>(e->scratch.as<unsigned char>() + tbytes);
     NBX_HIP(error, hipMemcpyAsync(dt, targets, count * sizeof(int64_t), hipMemcpyHostToDevice,
                                   e->stream));
-    nbx::direct_fp64_kernel<true>
-        <<<nbx::grid_for(count, nbx::kExactTargets), nbx::kExactThreads, 0, e->stream>>>(
-            e->px, e->py, e->pz, e->m, nullptr, nullptr, nullptr, static_cast<int>(e->n),
-            gravity->gravitational_constant, gravity->softening_squared, dt,
-            static_cast<int>(count), dout);
+    nbx::launch_direct_fp64<true>(e->stream, e->cu_count, e->px, e->py, e->pz, e->m, nullptr,
+                                  nullptr, nullptr, static_cast<int>(e->n),
+                                  gravity->gravitational_constant, gravity->softening_squared, dt,
+                                  static_cast<int>(count), dout);
     NBX_HIP(error, hipGetLastError());
     NBX_HIP(error, hipMemcpyAsync(out, dout, 3 * count * sizeof(double), hipMemcpyDeviceToHost,
                                   e->stream));

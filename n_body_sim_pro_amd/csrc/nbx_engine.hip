@@ -358,6 +358,12 @@ NBXEngine* nbx_engine_create(const NBXConfig* config, NBodySimProError* error) {
     e->flags = config->flags;
     bool ok = hipSetDevice(e->device) == hipSuccess &&
               hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) == hipSuccess;
+    int cus = 0;
+    if (ok && hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, e->device) ==
+                  hipSuccess &&
+        cus > 0) {
+        e->cu_count = cus;
+    }
     double** arrays[] = {&e->px, &e->py, &e->pz, &e->vx, &e->vy, &e->vz,
                          &e->ax, &e->ay, &e->az, &e->m};
     for (double** a : arrays) {

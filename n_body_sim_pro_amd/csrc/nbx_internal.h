@@ -142,6 +142,7 @@ struct BHState {
 
 struct Engine {
     int device = 0;
+    int cu_count = 256;       // compute units of the device (grid shape choices)
     hipStream_t stream = nullptr;
     size_t capacity = 0;
     size_t n = 0;

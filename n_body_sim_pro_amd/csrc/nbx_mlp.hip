@@ -695,13 +695,16 @@ __global__ __launch_bounds__(kMlpThreads) void learned_apply_kernel(
         if (valid) {
             const float y0 = ys[lane][0], y1 = ys[lane][1], y2 = ys[lane][2];
             const uint32_t o = tperm[t];
-            // the output is in units of the walk's quadrupole force |q|
+            // the walk's quadrupole force q plus the network's estimate of
+            // what q leaves, in units of |q|
+            const float* wf = walkf + static_cast<size_t>(t) * kWalkRecord;
             const double qn = sqrt(static_cast<double>(fr.q_par) * fr.q_par +
                                    static_cast<double>(fr.q_perp) * fr.q_perp);
             const double k = static_cast<double>(G) * qn / out_scale;
-            ax[o] = bx + k * (y0 * fr.e1[0] + y1 * fr.e2[0] + y2 * fr.e3[0]);
-            ay[o] = by + k * (y0 * fr.e1[1] + y1 * fr.e2[1] + y2 * fr.e3[1]);
-            az[o] = bz + k * (y0 * fr.e1[2] + y1 * fr.e2[2] + y2 * fr.e3[2]);
+            const double gq = static_cast<double>(G);
+            ax[o] = bx + gq * wf[4] + k * (y0 * fr.e1[0] + y1 * fr.e2[0] + y2 * fr.e3[0]);
+            ay[o] = by + gq * wf[5] + k * (y0 * fr.e1[1] + y1 * fr.e2[1] + y2 * fr.e3[1]);
+            az[o] = bz + gq * wf[6] + k * (y0 * fr.e1[2] + y1 * fr.e2[2] + y2 * fr.e3[2]);
         }
         wave_lds_sync();
     }

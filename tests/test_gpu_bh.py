@@ -271,3 +271,32 @@ def test_bh_phase_timings_are_consistent(nbx):
             assert all(t[k] > 0.0 for k in phases), (alg, t)
             assert sum(t[k] for k in phases) <= t["force_ms"] * 1.001 + 1e-3, (alg, t)
             assert t["kernel_ms"] == t["walk_ms"]
+
+
+@pytest.mark.parametrize("theta", [0.7, 1.0])
+def test_member_test_still_refines_reference_lists(nbx, theta):
+    """NBX_FLAG_BH_MEMBER_TEST: a cell the group's box test opens is accepted
+    when every member's own test accepts it. The shared lists shrink (no more
+    interactions than the box test) but still refine every member's reference
+    list (no fewer than the reference lists), and the accuracy against the
+    FP64 direct sum stays at least the reference lists'."""
+    n = 200_000
+    d = nbx.two_galaxy(n, seed=42)
+    idx = np.sort(np.random.default_rng(3).choice(n, 2048, replace=False))
+    out = {}
+    for key, kw in (("reference", {"reference_lists": True}), ("box", {}),
+                    ("member", {"member_test": True})):
+        with nbx.Engine(n, theta=theta, **kw) as e:
+            e.upload(d["pos"], d["m"])
+            e.compute_force("hip_barnes_hut", eps=0.01)
+            s = e.stats()
+            acc = e.accelerations()
+            ref = e.direct_fp64_targets(idx, 1.0, 0.01)
+        err = np.linalg.norm(acc[idx] - ref, axis=1) / np.linalg.norm(ref, axis=1)
+        out[key] = (s["accepted_approximations"] + s["exact_interactions"], np.median(err))
+        print(f"theta={theta} {key}: interactions {out[key][0]} median err {out[key][1]:.3e}")
+    assert out["reference"][0] <= out["member"][0] < out["box"][0]
+    # nested lists: the box lists refine the member lists, which refine the
+    # reference lists (medians of 2048 sampled particles, 2% slack)
+    assert out["box"][1] <= 1.02 * out["member"][1]
+    assert out["member"][1] <= 1.02 * out["reference"][1]

@@ -243,13 +243,14 @@ def test_feature_walk_matches_numpy_restatement(nbx):
 
 
 def test_group_walk_features_are_consistent(nbx):
-    """Default engine: the feature walk is the group walk. Its accelerations are
-    the plain group walk's up to FP32 summation order (it flushes its staged
-    lists in smaller batches), the per-particle counts add up to the walk's
-    totals (every member of a group shares the list length), frames are
-    orthonormal and the quadrupole features are consistent."""
+    """Default engine: the feature walk is the group walk with the member
+    test. Its accelerations are the plain group walk's with that test up to
+    FP32 summation order (it flushes its staged lists in smaller batches), the
+    per-particle counts add up to the walk's totals (every member of a group
+    shares the list length), frames are orthonormal and the quadrupole
+    features are consistent."""
     d = nbx.two_galaxy(50_000, seed=8)
-    with nbx.Engine(50_000, theta=1.0) as e:
+    with nbx.Engine(50_000, theta=1.0, member_test=True) as e:
         e.upload(d["pos"], d["m"])
         feats, frames = e.bh_features(1.0, 0.01, frames=True)
         acc_feat = e.accelerations()
@@ -295,8 +296,9 @@ def test_correction_kernel_matches_host_formula(nbx, learned):
         a_l = e.accelerations()
     a = np.linalg.norm(a_bh, axis=1)
     qn = learned.quad_norm(a_bh, feats)
-    want = a_bh + (qn / out_scale)[:, None] * np.einsum("ni,nij->nj", y.astype(np.float64),
-                                                         frames.astype(np.float64))
+    q = learned.quad_force(a_bh, feats, frames)
+    want = a_bh + q + (qn / out_scale)[:, None] * np.einsum("ni,nij->nj", y.astype(np.float64),
+                                                             frames.astype(np.float64))
     corr = np.linalg.norm(want - a_bh, axis=1)
     assert np.median(corr / a) > 1e-3                    # the correction is not trivial
     err = np.linalg.norm(a_l - want, axis=1) / a
@@ -306,8 +308,9 @@ def test_correction_kernel_matches_host_formula(nbx, learned):
 
 def test_without_model_hip_bh_learned_is_plain_bh(nbx, learned):
     """No model: hip_bh_learned is hip_barnes_hut, bit for bit (default group
-    walk and reference lists alike). A zero model leaves the corrected path's
-    base, the per-particle walk with reference lists, unchanged."""
+    walk and reference lists alike). A zero model gives the corrected path's
+    base, here the per-particle walk with reference lists, plus its quadrupole
+    force and nothing else."""
     d = nbx.two_galaxy(30000, seed=3)
     for ref_lists in (False, True):
         with nbx.Engine(30000, theta=1.0, reference_lists=ref_lists) as e:
@@ -317,21 +320,26 @@ def test_without_model_hip_bh_learned_is_plain_bh(nbx, learned):
             e.compute_force("hip_bh_learned", 1.0, 0.01)
             assert np.array_equal(e.accelerations(), plain)
             if ref_lists:
+                feats, frames = e.bh_features(1.0, 0.01, frames=True)
+                assert np.array_equal(e.accelerations(), plain)
+                q = learned.quad_force(plain, feats, frames)
                 with learned.Mlp() as m:
                     m.set_params(np.zeros(m.n_params), np.zeros(16), np.ones(16), 1.0)
                     e.set_mlp(m)
                     e.compute_force("hip_bh_learned", 1.0, 0.01)
-                    assert np.array_equal(e.accelerations(), plain)
+                    dq = e.accelerations() - plain - q
+                    qn = np.linalg.norm(q, axis=1)
+                    assert np.all(np.linalg.norm(dq, axis=1) <= 1e-5 * qn + 1e-12)
                     e.set_mlp(None)
             e.compute_force("hip_bh_learned", 1.0, 0.01)
             assert np.array_equal(e.accelerations(), plain)
 
 
 def _errors(nbx, learned, d, mlp, eps=0.01):
-    """Plain group walk (the corrected path's base) vs the corrected walk, both
-    against the FP64 direct sum."""
+    """The corrected path's base walk (the group walk with the member test)
+    vs the corrected walk, both against the FP64 direct sum."""
     n = len(d["m"])
-    with nbx.Engine(n, theta=1.0) as e:
+    with nbx.Engine(n, theta=1.0, member_test=True) as e:
         e.upload(d["pos"], d["m"])
         e.compute_force("hip_direct_fp64", 1.0, eps)
         ref = e.accelerations()
@@ -387,3 +395,31 @@ def test_committed_model_beats_theta05(nbx, learned):
     e_corr = np.median(np.linalg.norm(corr - ref, axis=1) / r)
     print(f"median error: plain theta=0.5 {e_half:.4e}, theta=1.0 + model {e_corr:.4e}")
     assert e_corr < e_half
+
+
+def test_network_gain_over_the_quadrupole_alone(nbx, learned):
+    """What the network adds: on a held-out 2M two-galaxy collision (neither
+    the size nor the system is in the training set), theta = 1.0 with the
+    quadrupole force and the network has a lower median error (FP64 direct
+    sum on 8192 seeded particles) than the same walk with the quadrupole force
+    alone, i.e. the zero model."""
+    n = 2_000_000
+    d = nbx.two_galaxy(n, seed=44)
+    idx = np.sort(np.random.default_rng(7).choice(n, 8192, replace=False))
+    mlp = learned.Mlp.load(learned.WEIGHTS)
+    with nbx.Engine(n, theta=1.0) as e, learned.Mlp() as zero:
+        zero.set_params(np.zeros(zero.n_params), np.zeros(16), np.ones(16), 1.0)
+        e.upload(d["pos"], d["m"])
+        ref = e.direct_fp64_targets(idx, 1.0, 0.01)
+        out = {}
+        for name, model in (("quadrupole", zero), ("network", mlp)):
+            e.set_mlp(model)
+            e.compute_force("hip_bh_learned", 1.0, 0.01)
+            a = e.accelerations()[idx]
+            out[name] = float(np.median(np.linalg.norm(a - ref, axis=1)
+                                        / np.linalg.norm(ref, axis=1)))
+        e.set_mlp(None)
+    mlp.close()
+    print(f"median error: quadrupole alone {out['quadrupole']:.4e}, with the network "
+          f"{out['network']:.4e} (ratio {out['network'] / out['quadrupole']:.3f})")
+    assert out["network"] < out["quadrupole"]
