@@ -1364,7 +1364,6 @@ __global__ __launch_bounds__(kGroupThreads) void group_walk_kernel(
     const bool valid = t < n;
     const float4 q = s4[valid ? t : n - 1];
     const float qx = q.x, qy = q.y, qz = q.z;
-    const uint64_t lt = lane == 0 ? 0ull : ((~0ull) >> (kWave - lane));
     WalkAcc<FEATURES> acc;
     bool per_query = false;   // this lane's sub-group could not share a list
 
@@ -1510,7 +1509,7 @@ __global__ __launch_bounds__(kGroupThreads) void group_walk_kernel(
             }
             const uint64_t cbal = __ballot(accepted);
             if (accepted) {
-                const int pos = ccnt + static_cast<int>(__popcll(cbal & lt));
+                const int pos = ccnt + static_cast<int>(lanes_below(cbal));
                 lcell[kCellWords * pos] = c0;
                 if (FEATURES) {
                     lcell[kCellWords * pos + 1] = gq0;
@@ -1533,7 +1532,7 @@ __global__ __launch_bounds__(kGroupThreads) void group_walk_kernel(
                     }
                     if (ch < -1) {
                         const int tag = -ch - 2;
-                        ltag[lcnt + static_cast<int>(__popcll(lb & lt))] = tag;
+                        ltag[lcnt + static_cast<int>(lanes_below(lb))] = tag;
                         const unsigned rel = static_cast<unsigned>(tag - t0);
                         if (rel < static_cast<unsigned>(kWave)) {
                             selfhit[rel] = 1;
@@ -1546,7 +1545,7 @@ __global__ __launch_bounds__(kGroupThreads) void group_walk_kernel(
                     overflow = true;
                 } else if (np != 0) {
                     if (ch >= 0) {
-                        stack[sp + static_cast<int>(__popcll(ib & lt))] = ch;
+                        stack[sp + static_cast<int>(lanes_below(ib))] = ch;
                     }
                     sp += np;
                 }

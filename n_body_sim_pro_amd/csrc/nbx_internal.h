@@ -190,6 +190,13 @@ NBodySimProStatus radix_sort_pairs(hipStream_t s, uint64_t* keys, uint32_t* vals
 NBodySimProStatus exclusive_scan_u32(hipStream_t s, const uint32_t* in, uint32_t* out, size_t n,
                                      Buf& partial, NBodySimProError* error);
 
+// Bits of `mask` set in the lanes below this one (v_mbcnt_lo/hi: two VALU
+// instructions, where and-ing a per-lane mask and counting bits takes four).
+__device__ inline unsigned lanes_below(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi(static_cast<unsigned>(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(mask), 0u));
+}
+
 inline unsigned grid_for(size_t n, unsigned block) {
     return static_cast<unsigned>((n + block - 1) / block);
 }

@@ -161,7 +161,6 @@ __global__ __launch_bounds__(kSortThreads) void sort_scatter(
 
     // Stable ranks: rounds in input order, waves in order within a round,
     // lanes in order within a wave.
-    const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (kWave - lane));
     const size_t tbase = static_cast<size_t>(tile) * kSortTile;
     for (int r = 0; r < kSortIpt; ++r) {
         const size_t i = tbase + static_cast<size_t>(r) * kSortThreads + tid;
@@ -181,7 +180,7 @@ __global__ __launch_bounds__(kSortThreads) void sort_scatter(
             const uint64_t bal = __ballot(bit);
             peers &= bit ? bal : ~bal;
         }
-        const unsigned rank = __popcll(peers & lt);
+        const unsigned rank = lanes_below(peers);
         const bool leader = valid && rank == 0;
         const uint32_t group = static_cast<uint32_t>(__popcll(peers));
         if (leader) {
