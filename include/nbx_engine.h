@@ -62,20 +62,23 @@ enum {
     /* hip_barnes_hut walks every particle with its own reference interaction
      * list (the reference's acceptance per particle, reference counts in the
      * stats) instead of the default group walk, in which each wave of 64
-     * consecutive particles shares one list accepted against the group's
-     * bounding box (every particle's list then refines its reference list:
-     * more interactions, same or smaller error). */
+     * consecutive particles shares one list of the cells accepted against the
+     * group's bounding box or by every member (every particle's list then
+     * refines its reference list: more interactions, same or smaller error). */
     NBX_FLAG_BH_REFERENCE_LISTS = 1u,
     /* group walk only: a group of 64 that is not compact (its smallest
      * enclosing cell holds more than 2048 particles) walks each of its
      * particles with its reference list instead of being split into compact
      * parts that each share a list. Slower; for comparisons and tests. */
     NBX_FLAG_BH_NO_SPLIT = 2u,
-    /* group walk only: a cell that the group's bounding-box test opens is
-     * still accepted when every member's own reference test accepts it
-     * (fewer interactions; each list still refines its members' reference
-     * lists). */
-    NBX_FLAG_BH_MEMBER_TEST = 4u
+    /* group walk: a cell that the group's bounding-box test opens is still
+     * accepted when every member's own reference test accepts it (fewer
+     * interactions; each list still refines its members' reference lists).
+     * This is the group walk's default; the bit only states it. */
+    NBX_FLAG_BH_MEMBER_TEST = 4u,
+    /* group walk only: the group's bounding-box test alone (no member test):
+     * longer lists, same or smaller error. For comparisons and tests. */
+    NBX_FLAG_BH_BOX_TEST = 8u
 };
 
 typedef struct NBXConfig {

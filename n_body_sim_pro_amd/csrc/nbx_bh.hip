@@ -1484,25 +1484,32 @@ __global__ __launch_bounds__(kGroupThreads) void group_walk_kernel(
                     const float fz = fmaxf(fabsf(c0.z - bz0), fabsf(c0.z - bz1));
                     cand = c1.x < theta2 * fmaf(fx, fx, fmaf(fy, fy, fz * fz));
                 }
-                uint64_t pending = __ballot(cand);
-                uint64_t rescued = 0;
-                while (pending != 0) {
-                    const int l = __builtin_ctzll(pending);
-                    pending &= pending - 1;
-                    const float cx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(c0.x), l));
-                    const float cy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(c0.y), l));
-                    const float cz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(c0.z), l));
-                    const float cs = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(c1.x), l));
-                    const float dx = cx - qx;
-                    const float dy = cy - qy;
-                    const float dz = cz - qz;
-                    const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                    const bool fails = member && !(d2 > 0.f && cs < theta2 * d2);
-                    if (__ballot(fails) == 0) {
-                        rescued |= 1ull << l;
+                // The candidates go to LDS in lane order (the staged-leaf
+                // buffer: it only holds data inside a flush) and every member
+                // reads each one by broadcast; the tests take no branches.
+                const uint64_t pending = __ballot(cand);
+                if (pending != 0) {
+                    const int slot = static_cast<int>(lanes_below(pending));
+                    if (cand) {
+                        lleaf[slot] = make_float4(c0.x, c0.y, c0.z, c1.x);
                     }
+                    wave_lds_fence();
+                    const int nc = static_cast<int>(__popcll(pending));
+                    uint64_t pass = 0;   // bit j: every member accepts candidate j
+                    for (int j = 0; j < nc; ++j) {
+                        const float4 cj = lleaf[j];
+                        const float dx = cj.x - qx;
+                        const float dy = cj.y - qy;
+                        const float dz = cj.z - qz;
+                        const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                        const bool ok = (d2 > 0.f) & (cj.w < theta2 * d2);
+                        if (__ballot(member & !ok) == 0) {
+                            pass |= 1ull << j;
+                        }
+                    }
+                    wave_lds_fence();   // candidates read before the buffer is reused
+                    accepted = accepted || (cand && ((pass >> slot) & 1ull) != 0);
                 }
-                accepted = accepted || ((rescued >> lane) & 1ull) != 0;
             }
             if (ccnt > kList - kWave) {
                 flush();
@@ -2003,14 +2010,14 @@ NBodySimProStatus bh_walk(Engine* e, const NBodySimProGravity* g, bool fp64_exac
         unsigned long long* cnts = B<unsigned long long>(s.counters);
         // one shared walk per group (no split) or the split's budget
         const int max_walks = (e->flags & NBX_FLAG_BH_NO_SPLIT) ? 1 : kMaxSubWalks;
-        // The feature walk of the learned path always takes the member test:
-        // its accepted cells cost about three times a plain walk's (the
-        // quadrupole force), so the cells it saves outweigh the test.
-        // (Measured on MI355X, 10M two-galaxy: feature walk at theta = 1.0
-        // 9.61 -> 8.13 ms; the plain walk at theta = 0.7 0.90 -> 0.96 ms at
-        // 1M and at theta = 0.5 18.6 -> 19.2 ms at 10M, so it stays opt-in.)
+        // The member test is on unless the box test alone is asked for; the
+        // feature walk of the learned path always takes it. (Measured on
+        // MI355X, two-galaxy, box test alone -> with the member test: plain
+        // walk at theta = 0.7, 1M: 0.844 -> 0.819 ms, 1.53G -> 1.18G
+        // interactions; theta = 0.5, 10M: 18.3 -> 17.2 ms; feature walk at
+        // theta = 1.0, 10M: 9.61 -> 7.96 ms.)
         const bool member_test =
-            features != nullptr || (e->flags & NBX_FLAG_BH_MEMBER_TEST) != 0;
+            features != nullptr || (e->flags & NBX_FLAG_BH_BOX_TEST) == 0;
         auto grid_q = [&](int q) {
             const unsigned blocks_q = grid_for(s.n, kGroupThreads * q);
             return (blocks_q + kXcds - 1) / kXcds * kXcds;

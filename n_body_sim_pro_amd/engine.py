@@ -120,17 +120,18 @@ class Engine:
     """Device-resident engine (one HIP stream and its buffers on one device)."""
 
     def __init__(self, capacity, device=0, theta=0.7, reference_lists=False, split=True,
-                 member_test=False):
+                 member_test=True):
         """reference_lists: hip_barnes_hut walks each particle with its own
         reference interaction list (NBX_FLAG_BH_REFERENCE_LISTS) instead of the
         default group walk. split=False: groups that are not compact walk per
-        particle instead of being split (NBX_FLAG_BH_NO_SPLIT). member_test:
-        the group walk also accepts a cell that every member's own test
-        accepts (NBX_FLAG_BH_MEMBER_TEST)."""
+        particle instead of being split (NBX_FLAG_BH_NO_SPLIT). member_test
+        (default): the group walk also accepts a cell that every member's own
+        test accepts; False keeps the group's box test alone
+        (NBX_FLAG_BH_BOX_TEST)."""
         self.lib = load()
         cfg = EngineConfig(capacity=int(capacity), device=int(device), theta=float(theta),
                            flags=(1 if reference_lists else 0) | (0 if split else 2)
-                           | (4 if member_test else 0))
+                           | (0 if member_test else 8))
         err = Error()
         self.h = self.lib.nbx_engine_create(C.byref(cfg), C.byref(err))
         if not self.h:

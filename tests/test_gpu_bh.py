@@ -161,26 +161,32 @@ def test_bh_two_galaxy_fp32_inputs_keys_order_topology(nbx, oracle):
 
 @pytest.mark.parametrize("n", [200_000, 4_500_000])
 def test_group_walk_refines_reference_lists(nbx, oracle, n):
-    """The default FP32 group walk accepts a cell only when the whole group's
-    box passes the reference test, so it makes at least the reference's
-    interactions and is at least as accurate (FP64 direct sum on a sample)."""
+    """The FP32 group walk accepts a cell only when the whole group's box
+    passes the reference test (NBX_FLAG_BH_BOX_TEST) or, by default, also when
+    every member's own reference test passes, so it makes at least the
+    reference's interactions and is at least as accurate (FP64 direct sum on
+    a sample; the default lists are closest to the reference lists, so their
+    median gets 2% slack)."""
     d = nbx.two_galaxy(n, seed=42)
     idx = np.sort(np.random.default_rng(3).choice(n, 2048, replace=False))
     out = {}
-    for ref_lists in (True, False):
-        with nbx.Engine(n, theta=0.7, reference_lists=ref_lists) as e:
+    for key, kw in (("reference", {"reference_lists": True}), ("default", {}),
+                    ("box", {"member_test": False})):
+        with nbx.Engine(n, theta=0.7, **kw) as e:
             e.upload(d["pos"], d["m"])
             e.compute_force("hip_barnes_hut", eps=0.01)
             s = e.stats()
             acc = e.accelerations()
             ref = e.direct_fp64_targets(idx, 1.0, 0.01)
         err = np.linalg.norm(acc[idx] - ref, axis=1) / np.linalg.norm(ref, axis=1)
-        out[ref_lists] = (s["accepted_approximations"] + s["exact_interactions"], err)
-        print(f"reference_lists={ref_lists}: interactions {out[ref_lists][0]} "
+        out[key] = (s["accepted_approximations"] + s["exact_interactions"], err)
+        print(f"{key}: interactions {out[key][0]} "
               f"median err {np.median(err):.3e} p99 {np.quantile(err, 0.99):.3e}")
-    assert out[False][0] >= out[True][0]
-    assert np.median(out[False][1]) <= np.median(out[True][1])
-    assert np.quantile(out[False][1], 0.99) <= 1.05 * np.quantile(out[True][1], 0.99)
+    assert out["box"][0] > out["default"][0] >= out["reference"][0]
+    assert np.median(out["box"][1]) <= np.median(out["reference"][1])
+    assert np.quantile(out["box"][1], 0.99) <= 1.05 * np.quantile(out["reference"][1], 0.99)
+    assert np.median(out["default"][1]) <= 1.02 * np.median(out["reference"][1])
+    assert np.quantile(out["default"][1], 0.99) <= 1.05 * np.quantile(out["reference"][1], 0.99)
     # the group walk visits the same tree: momentum of the force field stays ~0
     with nbx.Engine(n, theta=0.7) as e:
         e.upload(d["pos"], d["m"])
@@ -275,17 +281,18 @@ def test_bh_phase_timings_are_consistent(nbx):
 
 @pytest.mark.parametrize("theta", [0.7, 1.0])
 def test_member_test_still_refines_reference_lists(nbx, theta):
-    """NBX_FLAG_BH_MEMBER_TEST: a cell the group's box test opens is accepted
-    when every member's own test accepts it. The shared lists shrink (no more
-    interactions than the box test) but still refine every member's reference
-    list (no fewer than the reference lists), and the accuracy against the
-    FP64 direct sum stays at least the reference lists'."""
+    """The group walk's member test (its default): a cell the group's box
+    test opens is accepted when every member's own test accepts it. The
+    shared lists shrink (fewer interactions than the box test alone,
+    NBX_FLAG_BH_BOX_TEST) but still refine every member's reference list (no
+    fewer than the reference lists), and the accuracy against the FP64 direct
+    sum stays at least the reference lists'."""
     n = 200_000
     d = nbx.two_galaxy(n, seed=42)
     idx = np.sort(np.random.default_rng(3).choice(n, 2048, replace=False))
     out = {}
-    for key, kw in (("reference", {"reference_lists": True}), ("box", {}),
-                    ("member", {"member_test": True})):
+    for key, kw in (("reference", {"reference_lists": True}), ("box", {"member_test": False}),
+                    ("member", {})):
         with nbx.Engine(n, theta=theta, **kw) as e:
             e.upload(d["pos"], d["m"])
             e.compute_force("hip_barnes_hut", eps=0.01)
