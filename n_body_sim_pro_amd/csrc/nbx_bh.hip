@@ -418,7 +418,7 @@ constexpr int kTopoWaves = kThreads / 64;
 // Internal cells by level index.
 struct InternalSoA {
     double4* com;     // FP64 centre of mass (x, y, z) and mass
-    int4* rec;        // node index, skip (node index past the subtree), first tree position, particles
+    int4* rec;        // node index, skip (node index past the subtree), first and last tree position
     int32_t* depth;
 };
 
@@ -581,16 +581,20 @@ __device__ inline void depth_offsets(const unsigned* __restrict__ dcount, unsign
 // Per-tile rank bookkeeping shared by ends_kernel and fill_kernel: the tile is
 // walked in chunks of kThreads positions; per chunk each wave records how many
 // of its lanes hold each depth (wcount), and `running` carries the counts of
-// the tile's earlier chunks.
+// the tile's earlier chunks. `base` holds this tile's row of the per-depth
+// counts of the earlier tiles (read once per workgroup).
 struct TileRanks {
     unsigned running[kMaxLevels];
     unsigned wcount[kTopoWaves][kMaxLevels];
     unsigned doff[kMaxLevels];
+    unsigned base[kMaxLevels];
 };
 
-__device__ inline void tile_ranks_init(TileRanks& r, const unsigned* __restrict__ dcount) {
+__device__ inline void tile_ranks_init(TileRanks& r, const unsigned* __restrict__ dcount,
+                                       const unsigned* __restrict__ base_row, int tiles) {
     for (int d = threadIdx.x; d < kMaxLevels; d += kThreads) {
         r.running[d] = 0;
+        r.base[d] = base_row[static_cast<size_t>(d) * tiles + blockIdx.x];
     }
     depth_offsets(dcount, r.doff);
 }
@@ -625,28 +629,30 @@ __device__ inline void tile_ranks_advance(TileRanks& r) {
 
 // Rank, among the holders of depth d, of the lanes before this one: earlier
 // tiles, earlier chunks, earlier waves of this chunk, earlier lanes.
-__device__ inline unsigned tile_rank_before(const TileRanks& r, const unsigned* __restrict__ base_row,
-                                            int tiles, int d, unsigned long long mask) {
+__device__ inline unsigned tile_rank_before(const TileRanks& r, int d, unsigned long long mask) {
     const int wv = threadIdx.x >> 6;
-    unsigned before = base_row[static_cast<size_t>(d) * tiles + blockIdx.x] + r.running[d];
+    unsigned before = r.base[d] + r.running[d];
     for (int k = 0; k < wv; ++k) {
         before += r.wcount[k][d];
     }
-    const unsigned lane = threadIdx.x & 63;
-    return before + static_cast<unsigned>(__popcll(mask & ((1ull << lane) - 1ull)));
+    return before + lanes_below(mask);
 }
 
-// endpos[L] = last tree position of cell L.
+// The end-side fields of every cell L's record, from its last tree position
+// e: rec[L].y = skip (the node index past its subtree) and rec[L].w = e. The
+// fill writes the start-side fields (.x, .z) from the cell's first position.
 // Each cell's child table starts empty here (-1 in all eight slots); the
 // fill writes the occupied ones.
-__global__ __launch_bounds__(kThreads) void ends_kernel(const int32_t* __restrict__ delta, int n,
+__global__ __launch_bounds__(kThreads) void ends_kernel(const int32_t* __restrict__ delta,
+                                                       const uint32_t* __restrict__ cnt,
+                                                       const uint32_t* __restrict__ scan, int n,
                                                        int tiles,
                                                        const unsigned* __restrict__ dcount,
                                                        const unsigned* __restrict__ end_base,
-                                                       int* __restrict__ endpos,
+                                                       int4* __restrict__ rec,
                                                        GroupNode* __restrict__ gnode) {
     __shared__ TileRanks r;
-    tile_ranks_init(r, dcount);
+    tile_ranks_init(r, dcount, end_base, tiles);
     for (int i = 0; i < kTopoItems; ++i) {
         tile_ranks_clear_chunk(r);
         __syncthreads();
@@ -654,6 +660,8 @@ __global__ __launch_bounds__(kThreads) void ends_kernel(const int32_t* __restric
         const bool valid = e < n && e > 0;
         const int a = valid ? delta[e] + 1 : 0;
         const int b = valid ? delta[e - 1] : -1;
+        // node index past the subtree of every cell ending at e
+        const int skip = valid ? static_cast<int>(scan[e] + cnt[e]) + e + 1 : 0;
         const bool has = a <= b;
         const int wlo = wave_min_int(has ? a : 0x7FFFFFFF);
         const int whi = wave_max_int(has ? b : -1);
@@ -663,8 +671,10 @@ __global__ __launch_bounds__(kThreads) void ends_kernel(const int32_t* __restric
             const bool in = has && a <= d && d <= b;
             const unsigned long long m = __ballot(in);
             if (in) {
-                const unsigned L = r.doff[d] + tile_rank_before(r, end_base, tiles, d, m);
-                endpos[L] = e;
+                const unsigned L = r.doff[d] + tile_rank_before(r, d, m);
+                int* rl = reinterpret_cast<int*>(rec + L);
+                rl[1] = skip;
+                rl[3] = e;
                 int4* kids = reinterpret_cast<int4*>(gnode + L) + 2;
                 kids[0] = make_int4(-1, -1, -1, -1);
                 kids[1] = make_int4(-1, -1, -1, -1);
@@ -688,10 +698,9 @@ __global__ __launch_bounds__(kThreads) void fill_kernel(
     const uint32_t* __restrict__ scan, int n, int tiles, const double4* __restrict__ sp4,
     const double* __restrict__ root, InternalSoA in, NodeF64* __restrict__ n64,
     NodeF32* __restrict__ n32, const unsigned* __restrict__ dcount,
-    const unsigned* __restrict__ start_base, const int* __restrict__ endpos,
-    GroupNode* __restrict__ gnode) {
+    const unsigned* __restrict__ start_base, GroupNode* __restrict__ gnode) {
     __shared__ TileRanks r;
-    tile_ranks_init(r, dcount);
+    tile_ranks_init(r, dcount, start_base, tiles);
     const double root_h = root[3];
     for (int i = 0; i < kTopoItems; ++i) {
         tile_ranks_clear_chunk(r);
@@ -713,13 +722,13 @@ __global__ __launch_bounds__(kThreads) void fill_kernel(
             const unsigned long long m = __ballot(starts);
             if (valid && (starts || d == dprev)) {
                 // a start of its own, or the latest depth-dprev start before t
-                const int L = static_cast<int>(r.doff[d] + tile_rank_before(r, start_base, tiles, d, m)) -
+                const int L = static_cast<int>(r.doff[d] + tile_rank_before(r, d, m)) -
                               (starts ? 0 : 1);
                 if (starts) {
-                    const int e = endpos[L];
-                    const int idx = static_cast<int>(sc) + t + (d - lo);
-                    const int skip = static_cast<int>(scan[e] + cnt[e]) + e + 1;
-                    in.rec[L] = make_int4(idx, skip, t, e - t + 1);
+                    // the start-side fields; ends_kernel wrote .y and .w
+                    int* rl = reinterpret_cast<int*>(in.rec + L);
+                    rl[0] = static_cast<int>(sc) + t + (d - lo);
+                    rl[2] = t;
                     in.depth[L] = d;
                     if (parent >= 0) {
                         gnode[parent].kids[digit_at(w, ext, t, d - 1)] = L;
@@ -805,10 +814,10 @@ __device__ inline void com_node(int L, int depth, GroupNode* __restrict__ gnode,
         oy = p0.y;
         oz = p0.z;
     } else {
-        // the cell's particles are tree positions r.z .. r.z + r.w - 1
+        // the cell's particles are tree positions r.z .. r.w
         int minpos = r.z;
         uint32_t minrank = trank[r.z];
-        for (int t = r.z + 1; t < r.z + r.w; ++t) {
+        for (int t = r.z + 1; t <= r.w; ++t) {
             const uint32_t k = trank[t];
             if (k < minrank) {
                 minrank = k;
@@ -1877,7 +1886,6 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error,
     NBX_TRY(s.icom.need<double4>(internal + 1, error));
     NBX_TRY(s.irec.need<int4>(internal + 1, error));
     NBX_TRY(s.idepth.need<int32_t>(internal + 1, error));
-    NBX_TRY(s.endpos.need<int>(internal + 1, error));
     NBX_TRY(s.gnode.need<GroupNode>(internal + 1, error));
     // FP64 node records feed only the exact FP64 walk
     NodeF64* n64_out = nullptr;
@@ -1907,12 +1915,13 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error,
     s.ext_tree = ext_tree;
     const unsigned* start_base = B<unsigned>(s.tile_base);
     const unsigned* end_base = start_base + static_cast<size_t>(kMaxLevels) * tiles;
-    ends_kernel<<<nt, kThreads, 0, st>>>(B<int32_t>(s.delta), ni, nt, flags + kDCount, end_base,
-                                         B<int>(s.endpos), B<GroupNode>(s.gnode));
+    ends_kernel<<<nt, kThreads, 0, st>>>(B<int32_t>(s.delta), B<uint32_t>(s.cnt),
+                                         B<uint32_t>(s.scan), ni, nt, flags + kDCount, end_base,
+                                         in.rec, B<GroupNode>(s.gnode));
     fill_kernel<<<nt, kThreads, 0, st>>>(B<uint64_t>(s.tword), ext_tree, B<int32_t>(s.delta),
                                          B<uint32_t>(s.cnt), B<uint32_t>(s.scan), ni, nt, sp4,
                                          root, in, n64_out, B<NodeF32>(s.n32), flags + kDCount,
-                                         start_base, B<int>(s.endpos), B<GroupNode>(s.gnode));
+                                         start_base, B<GroupNode>(s.gnode));
     NBX_HIP(error, hipGetLastError());
     NBX_HIP(error, hipEventRecord(e->ev[3], st));
     std::vector<unsigned> offs(kMaxLevels, 0);

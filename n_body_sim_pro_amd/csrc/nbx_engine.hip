@@ -65,13 +65,13 @@ namespace {
 constexpr int kThreads = 256;
 
 // Every grow-on-demand device buffer of an engine (Barnes-Hut build and walk
-// arrays, diagnostics scratch).
-std::array<Buf*, 38> work_buffers(Engine* e) {
+// arrays, diagnostics scratch); the array's size follows the list.
+auto work_buffers(Engine* e) {
     BHState& s = e->bh;
-    return {&s.b4, &s.partial, &s.flags, &s.mkey, &s.tkey, &s.sk0, &s.sk1, &s.sv0, &s.sv1,
+    return std::array{&s.b4, &s.partial, &s.flags, &s.mkey, &s.tkey, &s.sk0, &s.sk1, &s.sv0, &s.sv1,
             &s.hist, &s.mperm, &s.mkey_sorted, &s.tperm, &s.trank, &s.tword, &s.ext, &s.delta,
             &s.cnt, &s.scan, &s.scan_part, &s.sp4, &s.s4, &s.tile_hist, &s.tile_base, &s.n64,
-            &s.n32, &s.icom, &s.irec, &s.idepth, &s.endpos, &s.gnode, &s.im2, &s.gquad,
+            &s.n32, &s.icom, &s.irec, &s.idepth, &s.gnode, &s.im2, &s.gquad,
             &s.gquad_pre, &s.tmp32, &s.counters, &e->scratch, &e->scratch2};
 }
 
@@ -700,7 +700,7 @@ NBodySimProStatus nbx_engine_bh_export_nodes(NBXEngine* engine, double* rows,
             m[k] = com[4 * i + 3];
             depth[k] = idepth[i];
             start[k] = rec[4 * i + 2];
-            count[k] = rec[4 * i + 3];
+            count[k] = rec[4 * i + 3] - rec[4 * i + 2] + 1;   // last - first + 1
         }
     }
     {

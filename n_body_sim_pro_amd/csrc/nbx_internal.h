@@ -131,7 +131,7 @@ struct BHState {
     // FP64 centre of mass and mass (double4), (node index, skip, first tree
     // position, particle count) (int4), depth, last tree position, GroupNode
     // (FP32 record and child table)
-    Buf icom, irec, idepth, endpos, gnode;
+    Buf icom, irec, idepth, gnode;
     // second moments about the centre of mass (FP64, 6 per cell) and the
     // walk's quadrupole records (2 float4 per cell, by level index and by
     // pre-order node index), learned path only
