@@ -14,8 +14,8 @@
  * component; it only counts accepted cells during its walk
  * (src/c/barnes_hut/barnes_hut.c:611), which is one of the features.
  *
- * The learned path's walk is the group walk with the member test of
- * NBX_FLAG_BH_MEMBER_TEST always on.
+ * The learned path's walk is the group walk with its member test, which stays
+ * on under NBX_FLAG_BH_BOX_TEST.
  *
  * Features (raw, before standardisation; all frame-free), per particle, with
  * q = sum over accepted cells of the quadrupole force 5/2 (d.Q.d) d / r^7 -
@@ -85,9 +85,8 @@ NBodySimProStatus nbx_engine_set_mlp(NBXEngine* engine, NBXMlp* mlp, NBodySimPro
 /* Build + walk (the learned path's walk) on the device-resident state,
  * returning the raw features (n x 16, original particle order) and, when
  * `frames` is not NULL, the local frames (n x 9: e1, e2, e3); the walk's
- * accelerations are left in the engine (as by hip_barnes_hut with
- * NBX_FLAG_BH_MEMBER_TEST; with NBX_FLAG_BH_REFERENCE_LISTS, its per-particle
- * walk). */
+ * accelerations are left in the engine (as by hip_barnes_hut's default group
+ * walk; with NBX_FLAG_BH_REFERENCE_LISTS, its per-particle walk). */
 NBodySimProStatus nbx_engine_bh_features(NBXEngine* engine, const NBodySimProGravity* gravity,
                                          float* features, float* frames, NBodySimProError* error);
 /* FP64 direct sum (the reference's scalar order) for selected targets only:
