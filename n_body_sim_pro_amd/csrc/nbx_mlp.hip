@@ -460,6 +460,7 @@ struct Mlp {
     Buf feats, targets, index, scratch_in, scratch_out;
     size_t dataset = 0;
     int cu_count = 256;
+    int apply_blocks_per_cu = 1;   // resident learned_apply_kernel workgroups per CU
 };
 
 int grad_waves(const Mlp& m, size_t n) {
@@ -721,9 +722,15 @@ NBodySimProStatus bh_learned(Engine* e, const NBodySimProGravity* g, NBodySimPro
     float* walkf = e->scratch2.as<float>();
     NBX_TRY(bh_walk(e, g, false, walkf, error));
     NBX_HIP(error, hipEventRecord(e->ev[7], e->stream));
+    // One resident round of workgroups that stride over the 64-query tiles:
+    // a grid beyond what fits leaves a second, partial round on some CUs.
+    // (Measured on MI355X at 10M: 4 workgroups per CU requested with 3
+    // resident, 0.86 ms; the resident count, 0.74 ms. Forcing 4 resident
+    // workgroups through a waves-per-EU bound spills 10 VGPRs: 0.81 ms.)
     const int tiles = static_cast<int>((s.n + kWave - 1) / kWave);
     int blocks = (tiles + kWavesPerBlock - 1) / kWavesPerBlock;
-    blocks = blocks < m->cu_count * 4 ? blocks : m->cu_count * 4;
+    const int resident = m->cu_count * m->apply_blocks_per_cu;
+    blocks = blocks < resident ? blocks : resident;
     learned_apply_kernel<<<blocks, kMlpThreads, 0, e->stream>>>(
         m->params, m->norm, m->norm + 2 * kF, m->out_scale, walkf, s.s4.as<float4>(),
         s.delta.as<int32_t>(), s.tperm.as<uint32_t>(), static_cast<int>(s.n),
@@ -811,6 +818,10 @@ NBXMlp* nbx_mlp_create(int device, NBodySimProError* error) {
               hipGetDeviceProperties(&prop, device) == hipSuccess &&
               hipStreamCreateWithFlags(&m.stream, hipStreamNonBlocking) == hipSuccess;
     if (ok) m.cu_count = prop.multiProcessorCount;
+    int per_cu = 0;
+    ok = ok && hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, nbx::learned_apply_kernel,
+                                                            nbx::kMlpThreads, 0) == hipSuccess;
+    if (ok) m.apply_blocks_per_cu = per_cu > 0 ? per_cu : 1;
     ok = ok && hipMalloc(reinterpret_cast<void**>(&m.params), nbx::kParams * sizeof(float)) == hipSuccess;
     ok = ok && hipMalloc(reinterpret_cast<void**>(&m.norm), 3 * nbx::kF * sizeof(float)) == hipSuccess;
     ok = ok && hipMalloc(reinterpret_cast<void**>(&m.grads), nbx::kPartial * sizeof(float)) == hipSuccess;
