@@ -1496,8 +1496,9 @@ __global__ __launch_bounds__(kGroupThreads) void group_walk_kernel(
                 // The candidates go to LDS in lane order (the staged-leaf
                 // buffer: it only holds data inside a flush) and every member
                 // reads each one by broadcast; the tests take no branches.
-                const uint64_t pending = __ballot(cand);
+                const uint64_t pending = __builtin_amdgcn_ballot_w64(cand);
                 if (pending != 0) {
+                    const uint64_t members = __builtin_amdgcn_ballot_w64(member);
                     const int slot = static_cast<int>(lanes_below(pending));
                     if (cand) {
                         lleaf[slot] = make_float4(c0.x, c0.y, c0.z, c1.x);
@@ -1511,8 +1512,11 @@ __global__ __launch_bounds__(kGroupThreads) void group_walk_kernel(
                         const float dy = cj.y - qy;
                         const float dz = cj.z - qz;
                         const float dj2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                        const bool ok = (dj2 > 0.f) & (cj.w < theta2 * dj2);
-                        if (__ballot(member & !ok) == 0) {
+                        // (each comparison balloted on its own: its lane
+                        // mask is the ballot, with no VALU round trip)
+                        const uint64_t ok = __builtin_amdgcn_ballot_w64(dj2 > 0.f) &
+                                            __builtin_amdgcn_ballot_w64(cj.w < theta2 * dj2);
+                        if ((members & ~ok) == 0) {
                             pass |= 1ull << j;
                         }
                     }
