@@ -142,6 +142,19 @@ def test_config2_1m_keys_order_topology_over_three_steps(nbx, oracle):
         _check_tree(e, oracle, s["pos"], d["m"])
 
 
+def test_large_input_keys_order_topology(nbx, oracle):
+    """Above 2^22 particles the radix sort takes 4096-item tiles instead of
+    2048: keys, sorted order, root cube and topology stay bit-exact against
+    the oracle's build (two-galaxy, N=4,300,000)."""
+    n = 4_300_000
+    d = nbx.two_galaxy(n, seed=11)
+    _say(f"large sort tiles: N={n}")
+    with nbx.Engine(n, theta=0.7) as e:
+        e.upload(d["pos"], d["m"])
+        e.compute_force("hip_barnes_hut", eps=0.01)
+        _check_tree(e, oracle, d["pos"], d["m"])
+
+
 def test_group_walk_unsoftened_refines_reference_lists(nbx):
     """eps = 0 (the reference's default softening): the group walk's leaf terms
     take the self-masked branch. With the box test alone and with the default
