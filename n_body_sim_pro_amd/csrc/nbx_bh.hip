@@ -578,31 +578,31 @@ __device__ inline void depth_offsets(const unsigned* __restrict__ dcount, unsign
     }
 }
 
-// Per-tile rank bookkeeping shared by ends_kernel and fill_kernel: the tile is
-// walked in chunks of kThreads positions; per chunk each wave records how many
-// of its lanes hold each depth (wcount), and `running` carries the counts of
-// the tile's earlier chunks. `base` holds this tile's row of the per-depth
-// counts of the earlier tiles (read once per workgroup).
+// Per-tile rank bookkeeping shared by ends_kernel and fill_kernel, which run
+// one tile per workgroup of kTopoTile threads (one position per thread):
+// each wave records how many of its lanes hold each depth (wcount), which
+// then becomes the exclusive prefix over the waves; `base` holds this tile's
+// row of the per-depth counts of the earlier tiles (read once per
+// workgroup). Three barriers per workgroup in all. (Measured on MI355X: the
+// earlier four 256-position chunks per workgroup, four barriers each, took
+// fill 544 and ends 184 us at 10M.)
+constexpr int kTopoBlockWaves = kTopoTile / 64;
+
 struct TileRanks {
-    unsigned running[kMaxLevels];
-    unsigned wcount[kTopoWaves][kMaxLevels];
+    unsigned wcount[kTopoBlockWaves][kMaxLevels];
     unsigned doff[kMaxLevels];
     unsigned base[kMaxLevels];
 };
 
 __device__ inline void tile_ranks_init(TileRanks& r, const unsigned* __restrict__ dcount,
                                        const unsigned* __restrict__ base_row, int tiles) {
-    for (int d = threadIdx.x; d < kMaxLevels; d += kThreads) {
-        r.running[d] = 0;
+    for (int k = threadIdx.x; k < kTopoBlockWaves * kMaxLevels; k += kTopoTile) {
+        (&r.wcount[0][0])[k] = 0;
+    }
+    for (int d = threadIdx.x; d < kMaxLevels; d += kTopoTile) {
         r.base[d] = base_row[static_cast<size_t>(d) * tiles + blockIdx.x];
     }
     depth_offsets(dcount, r.doff);
-}
-
-__device__ inline void tile_ranks_clear_chunk(TileRanks& r) {
-    for (int k = threadIdx.x; k < kTopoWaves * kMaxLevels; k += kThreads) {
-        (&r.wcount[0][0])[k] = 0;
-    }
 }
 
 // Lanes holding depth d (in [a, b]) counted per wave, for d in [wlo, whi].
@@ -616,26 +616,23 @@ __device__ inline void tile_ranks_count(TileRanks& r, int wlo, int whi, int a, i
     }
 }
 
-__device__ inline void tile_ranks_advance(TileRanks& r) {
-    for (int d = threadIdx.x; d < kMaxLevels; d += kThreads) {
-        unsigned s = 0;
+// wcount[w][d] := the holders of depth d in the waves before w.
+__device__ inline void tile_ranks_prefix(TileRanks& r) {
+    for (int d = threadIdx.x; d < kMaxLevels; d += kTopoTile) {
+        unsigned run = 0;
 #pragma unroll
-        for (int k = 0; k < kTopoWaves; ++k) {
-            s += r.wcount[k][d];
+        for (int k = 0; k < kTopoBlockWaves; ++k) {
+            const unsigned v = r.wcount[k][d];
+            r.wcount[k][d] = run;
+            run += v;
         }
-        r.running[d] += s;
     }
 }
 
 // Rank, among the holders of depth d, of the lanes before this one: earlier
-// tiles, earlier chunks, earlier waves of this chunk, earlier lanes.
+// tiles, earlier waves of this tile, earlier lanes.
 __device__ inline unsigned tile_rank_before(const TileRanks& r, int d, unsigned long long mask) {
-    const int wv = threadIdx.x >> 6;
-    unsigned before = r.base[d] + r.running[d];
-    for (int k = 0; k < wv; ++k) {
-        before += r.wcount[k][d];
-    }
-    return before + lanes_below(mask);
+    return r.base[d] + r.wcount[threadIdx.x >> 6][d] + lanes_below(mask);
 }
 
 // The end-side fields of every cell L's record, from its last tree position
@@ -643,46 +640,42 @@ __device__ inline unsigned tile_rank_before(const TileRanks& r, int d, unsigned 
 // fill writes the start-side fields (.x, .z) from the cell's first position.
 // Each cell's child table starts empty here (-1 in all eight slots); the
 // fill writes the occupied ones.
-__global__ __launch_bounds__(kThreads) void ends_kernel(const int32_t* __restrict__ delta,
-                                                       const uint32_t* __restrict__ cnt,
-                                                       const uint32_t* __restrict__ scan, int n,
-                                                       int tiles,
-                                                       const unsigned* __restrict__ dcount,
-                                                       const unsigned* __restrict__ end_base,
-                                                       int4* __restrict__ rec,
-                                                       GroupNode* __restrict__ gnode) {
+__global__ __launch_bounds__(kTopoTile) void ends_kernel(const int32_t* __restrict__ delta,
+                                                        const uint32_t* __restrict__ cnt,
+                                                        const uint32_t* __restrict__ scan, int n,
+                                                        int tiles,
+                                                        const unsigned* __restrict__ dcount,
+                                                        const unsigned* __restrict__ end_base,
+                                                        int4* __restrict__ rec,
+                                                        GroupNode* __restrict__ gnode) {
     __shared__ TileRanks r;
     tile_ranks_init(r, dcount, end_base, tiles);
-    for (int i = 0; i < kTopoItems; ++i) {
-        tile_ranks_clear_chunk(r);
-        __syncthreads();
-        const int e = blockIdx.x * kTopoTile + i * kThreads + threadIdx.x;
-        const bool valid = e < n && e > 0;
-        const int a = valid ? delta[e] + 1 : 0;
-        const int b = valid ? delta[e - 1] : -1;
-        // node index past the subtree of every cell ending at e
-        const int skip = valid ? static_cast<int>(scan[e] + cnt[e]) + e + 1 : 0;
-        const bool has = a <= b;
-        const int wlo = wave_min_int(has ? a : 0x7FFFFFFF);
-        const int whi = wave_max_int(has ? b : -1);
-        tile_ranks_count(r, wlo, whi, has ? a : 1, has ? b : 0);
-        __syncthreads();
-        for (int d = wlo; d <= whi; ++d) {
-            const bool in = has && a <= d && d <= b;
-            const unsigned long long m = __ballot(in);
-            if (in) {
-                const unsigned L = r.doff[d] + tile_rank_before(r, d, m);
-                int* rl = reinterpret_cast<int*>(rec + L);
-                rl[1] = skip;
-                rl[3] = e;
-                int4* kids = reinterpret_cast<int4*>(gnode + L) + 2;
-                kids[0] = make_int4(-1, -1, -1, -1);
-                kids[1] = make_int4(-1, -1, -1, -1);
-            }
+    const int e = blockIdx.x * kTopoTile + threadIdx.x;
+    const bool valid = e < n && e > 0;
+    const int a = valid ? delta[e] + 1 : 0;
+    const int b = valid ? delta[e - 1] : -1;
+    // node index past the subtree of every cell ending at e
+    const int skip = valid ? static_cast<int>(scan[e] + cnt[e]) + e + 1 : 0;
+    const bool has = a <= b;
+    const int wlo = wave_min_int(has ? a : 0x7FFFFFFF);
+    const int whi = wave_max_int(has ? b : -1);
+    __syncthreads();   // wcount cleared
+    tile_ranks_count(r, wlo, whi, has ? a : 1, has ? b : 0);
+    __syncthreads();
+    tile_ranks_prefix(r);
+    __syncthreads();
+    for (int d = wlo; d <= whi; ++d) {
+        const bool in = has && a <= d && d <= b;
+        const unsigned long long m = __ballot(in);
+        if (in) {
+            const unsigned L = r.doff[d] + tile_rank_before(r, d, m);
+            int* rl = reinterpret_cast<int*>(rec + L);
+            rl[1] = skip;
+            rl[3] = e;
+            int4* kids = reinterpret_cast<int4*>(gnode + L) + 2;
+            kids[0] = make_int4(-1, -1, -1, -1);
+            kids[1] = make_int4(-1, -1, -1, -1);
         }
-        __syncthreads();
-        tile_ranks_advance(r);
-        __syncthreads();
     }
 }
 
@@ -692,7 +685,7 @@ __global__ __launch_bounds__(kThreads) void ends_kernel(const int32_t* __restric
 // -(tree position + 2)). A position's cells are visited shallow to deep, so each one's parent
 // is the cell visited just before it: the depth-delta[t-1] cell that holds
 // t-1 and t (already started) for the shallowest, the previous one otherwise.
-__global__ __launch_bounds__(kThreads) void fill_kernel(
+__global__ __launch_bounds__(kTopoTile) void fill_kernel(
     const uint64_t* __restrict__ w, const uint64_t* __restrict__ ext,
     const int32_t* __restrict__ delta, const uint32_t* __restrict__ cnt,
     const uint32_t* __restrict__ scan, int n, int tiles, const double4* __restrict__ sp4,
@@ -702,73 +695,69 @@ __global__ __launch_bounds__(kThreads) void fill_kernel(
     __shared__ TileRanks r;
     tile_ranks_init(r, dcount, start_base, tiles);
     const double root_h = root[3];
-    for (int i = 0; i < kTopoItems; ++i) {
-        tile_ranks_clear_chunk(r);
-        __syncthreads();
-        const int t = blockIdx.x * kTopoTile + i * kThreads + threadIdx.x;
-        const bool valid = t < n;
-        const int dprev = valid && t > 0 ? delta[t - 1] : -1;
-        const int hi = valid ? delta[t] : -1;
-        const int lo = dprev + 1;
-        const bool has = valid && lo <= hi;
-        const int wlo = wave_min_int(valid ? (dprev > 0 ? dprev : 0) : 0x7FFFFFFF);
-        const int whi = wave_max_int(valid ? (dprev > hi ? dprev : hi) : -1);
-        tile_ranks_count(r, wlo, whi, has ? lo : 1, has ? hi : 0);
-        __syncthreads();
-        int parent = -1;
-        const uint32_t sc = valid ? scan[t] : 0u;
-        for (int d = wlo; d <= whi; ++d) {
-            const bool starts = has && lo <= d && d <= hi;
-            const unsigned long long m = __ballot(starts);
-            if (valid && (starts || d == dprev)) {
-                // a start of its own, or the latest depth-dprev start before t
-                const int L = static_cast<int>(r.doff[d] + tile_rank_before(r, d, m)) -
-                              (starts ? 0 : 1);
-                if (starts) {
-                    // the start-side fields; ends_kernel wrote .y and .w
-                    int* rl = reinterpret_cast<int*>(in.rec + L);
-                    rl[0] = static_cast<int>(sc) + t + (d - lo);
-                    rl[2] = t;
-                    in.depth[L] = d;
-                    if (parent >= 0) {
-                        gnode[parent].kids[digit_at(w, ext, t, d - 1)] = L;
-                    }
+    const int t = blockIdx.x * kTopoTile + threadIdx.x;
+    const bool valid = t < n;
+    const int dprev = valid && t > 0 ? delta[t - 1] : -1;
+    const int hi = valid ? delta[t] : -1;
+    const int lo = dprev + 1;
+    const bool has = valid && lo <= hi;
+    const int wlo = wave_min_int(valid ? (dprev > 0 ? dprev : 0) : 0x7FFFFFFF);
+    const int whi = wave_max_int(valid ? (dprev > hi ? dprev : hi) : -1);
+    __syncthreads();   // wcount cleared
+    tile_ranks_count(r, wlo, whi, has ? lo : 1, has ? hi : 0);
+    __syncthreads();
+    tile_ranks_prefix(r);
+    __syncthreads();
+    int parent = -1;
+    const uint32_t sc = valid ? scan[t] : 0u;
+    for (int d = wlo; d <= whi; ++d) {
+        const bool starts = has && lo <= d && d <= hi;
+        const unsigned long long m = __ballot(starts);
+        if (valid && (starts || d == dprev)) {
+            // a start of its own, or the latest depth-dprev start before t
+            const int L = static_cast<int>(r.doff[d] + tile_rank_before(r, d, m)) -
+                          (starts ? 0 : 1);
+            if (starts) {
+                // the start-side fields; ends_kernel wrote .y and .w
+                int* rl = reinterpret_cast<int*>(in.rec + L);
+                rl[0] = static_cast<int>(sc) + t + (d - lo);
+                rl[2] = t;
+                in.depth[L] = d;
+                if (parent >= 0) {
+                    gnode[parent].kids[digit_at(w, ext, t, d - 1)] = L;
                 }
-                parent = L;
             }
+            parent = L;
         }
-        if (valid) {
-            const int depth = 1 + (dprev > hi ? dprev : hi);
-            const int li = static_cast<int>(sc + cnt[t]) + t;
-            if (depth > 0 && parent >= 0) {
-                gnode[parent].kids[digit_at(w, ext, t, depth - 1)] = -(t + 2);
-            }
-            const double4 q = sp4[t];
-            if (n64 != nullptr) {   // FP64 records only for the exact FP64 walk
-                NodeF64 a;
-                a.cx = q.x;
-                a.cy = q.y;
-                a.cz = q.z;
-                a.m = q.w;
-                a.h = ldexp(root_h, -depth);
-                a.skip = li + 1;
-                a.leaf = t;
-                n64[li] = a;
-            }
-            NodeF32 b;
-            b.cx = static_cast<float>(q.x);
-            b.cy = static_cast<float>(q.y);
-            b.cz = static_cast<float>(q.z);
-            b.m = static_cast<float>(q.w);
-            b.size2 = 0.f;
-            b.skip = li + 1;
-            b.leaf = t;
-            b.inv_size2 = 0.f;
-            n32[li] = b;
+    }
+    if (valid) {
+        const int depth = 1 + (dprev > hi ? dprev : hi);
+        const int li = static_cast<int>(sc + cnt[t]) + t;
+        if (depth > 0 && parent >= 0) {
+            gnode[parent].kids[digit_at(w, ext, t, depth - 1)] = -(t + 2);
         }
-        __syncthreads();
-        tile_ranks_advance(r);
-        __syncthreads();
+        const double4 q = sp4[t];
+        if (n64 != nullptr) {   // FP64 records only for the exact FP64 walk
+            NodeF64 a;
+            a.cx = q.x;
+            a.cy = q.y;
+            a.cz = q.z;
+            a.m = q.w;
+            a.h = ldexp(root_h, -depth);
+            a.skip = li + 1;
+            a.leaf = t;
+            n64[li] = a;
+        }
+        NodeF32 b;
+        b.cx = static_cast<float>(q.x);
+        b.cy = static_cast<float>(q.y);
+        b.cz = static_cast<float>(q.z);
+        b.m = static_cast<float>(q.w);
+        b.size2 = 0.f;
+        b.skip = li + 1;
+        b.leaf = t;
+        b.inv_size2 = 0.f;
+        n32[li] = b;
     }
 }
 
@@ -1919,10 +1908,10 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error,
     s.ext_tree = ext_tree;
     const unsigned* start_base = B<unsigned>(s.tile_base);
     const unsigned* end_base = start_base + static_cast<size_t>(kMaxLevels) * tiles;
-    ends_kernel<<<nt, kThreads, 0, st>>>(B<int32_t>(s.delta), B<uint32_t>(s.cnt),
+    ends_kernel<<<nt, kTopoTile, 0, st>>>(B<int32_t>(s.delta), B<uint32_t>(s.cnt),
                                          B<uint32_t>(s.scan), ni, nt, flags + kDCount, end_base,
                                          in.rec, B<GroupNode>(s.gnode));
-    fill_kernel<<<nt, kThreads, 0, st>>>(B<uint64_t>(s.tword), ext_tree, B<int32_t>(s.delta),
+    fill_kernel<<<nt, kTopoTile, 0, st>>>(B<uint64_t>(s.tword), ext_tree, B<int32_t>(s.delta),
                                          B<uint32_t>(s.cnt), B<uint32_t>(s.scan), ni, nt, sp4,
                                          root, in, n64_out, B<NodeF32>(s.n32), flags + kDCount,
                                          start_base, B<GroupNode>(s.gnode));
