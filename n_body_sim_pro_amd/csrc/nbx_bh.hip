@@ -47,6 +47,7 @@
 // walk is a group walk over the level-indexed cells (group_walk_kernel).
 #include "nbx_internal.h"
 
+#include <cstring>
 #include <vector>
 
 namespace nbx {
@@ -1779,7 +1780,12 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error,
     // Tree order, neighbour depths and per-depth node counts for a given
     // ascending true-path order (null: Morton order), then one host read of
     // the flags, counts and root cube.
-    unsigned hflags[kFlagRead];
+    if (s.host_flags == nullptr) {
+        NBX_HIP(error, hipHostMalloc(&s.host_flags, kFlagRead * sizeof(unsigned) + 4 * sizeof(double),
+                                     hipHostMallocDefault));
+    }
+    unsigned* hflags = static_cast<unsigned*>(s.host_flags);
+    double* hroot = reinterpret_cast<double*>(hflags + kFlagRead);
     auto topology = [&](const uint32_t* asc_rank, const uint64_t* ext_asc, uint64_t* ext_t)
         -> NBodySimProStatus {
         tree_order_kernel<<<g, kThreads, 0, st>>>(
@@ -1796,16 +1802,23 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error,
         NBX_HIP(error, hipGetLastError());
         NBX_HIP(error, hipMemcpyAsync(hflags, flags, kFlagRead * sizeof(unsigned),
                                       hipMemcpyDeviceToHost, st));
-        NBX_HIP(error, hipMemcpyAsync(s.root, root, 4 * sizeof(double), hipMemcpyDeviceToHost,
+        NBX_HIP(error, hipMemcpyAsync(hroot, root, 4 * sizeof(double), hipMemcpyDeviceToHost,
                                       st));
         NBX_HIP(error, hipEventRecord(e->ev[10], st));
         // node numbering (internal nodes started per tree position, scanned)
         // runs on the device while the host waits for the flags; the copies
-        // into hflags and s.root land before this returns, on success or not
+        // go to pinned memory the engine owns, so they land whether or not
+        // this returns early
         const NBodySimProStatus scanned =
             exclusive_scan_u32(st, B<uint32_t>(s.cnt), B<uint32_t>(s.scan), n, s.scan_part,
                                error);
-        NBX_HIP(error, hipEventSynchronize(e->ev[10]));
+        // Poll rather than block: the device idles until the host has read
+        // the per-depth counts and enqueued the rest of the build.
+        hipError_t done;
+        while ((done = hipEventQuery(e->ev[10])) == hipErrorNotReady) {
+        }
+        NBX_HIP(error, done);
+        std::memcpy(s.root, hroot, sizeof(s.root));
         return scanned;
     };
     // The Morton order is the tree order unless some true-path key differs

@@ -401,6 +401,7 @@ void nbx_engine_destroy(NBXEngine* engine) {
     for (nbx::Buf* b : nbx::work_buffers(e)) {
         nbx::free_buf(*b);
     }
+    if (e->bh.host_flags != nullptr) (void)hipHostFree(e->bh.host_flags);
     if (e->mlp != nullptr) {
         nbx_mlp_release(e->mlp);
     }

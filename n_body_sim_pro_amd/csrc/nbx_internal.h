@@ -105,6 +105,9 @@ struct BHState {
     bool has_quad = false;    // the last build wrote quadrupole records
     const uint64_t* ext_tree = nullptr;   // tree-order extension words (tie path) or null
     double root[4] = {0, 0, 0, 1};
+    // pinned host block the build reads its flags, per-depth counts and root
+    // cube into (asynchronous copies; owned here so they always land)
+    void* host_flags = nullptr;
     // build inputs (FP64 x, y, z, m): either the master arrays or FP32-rounded copies
     Buf b4;                   // (x, y, z, m) per particle in original order
     Buf partial;              // bbox partials + root
@@ -128,9 +131,9 @@ struct BHState {
     Buf n64;                  // NodeF64
     Buf n32;                  // NodeF32
     // internal cells by level index (depth-major, tree order within a depth):
-    // FP64 centre of mass and mass (double4), (node index, skip, first tree
-    // position, particle count) (int4), depth, last tree position, GroupNode
-    // (FP32 record and child table)
+    // FP64 centre of mass and mass (double4), (node index, skip, first and
+    // last tree position) (int4), depth, GroupNode (FP32 record and child
+    // table)
     Buf icom, irec, idepth, gnode;
     // second moments about the centre of mass (FP64, 6 per cell) and the
     // walk's quadrupole records (2 float4 per cell, by level index and by
