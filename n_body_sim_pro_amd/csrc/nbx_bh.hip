@@ -247,10 +247,21 @@ __global__ __launch_bounds__(kThreads) void bbox_partial_kernel(
     }
 }
 
+// Also clears the build's flag words and the walk's interaction counters (no
+// fill launches of their own inside the evaluation).
 __global__ __launch_bounds__(kThreads) void bbox_final_kernel(const double* __restrict__ partial,
                                                              int blocks,
-                                                             double* __restrict__ root) {
+                                                             double* __restrict__ root,
+                                                             unsigned* __restrict__ flags,
+                                                             int flag_words,
+                                                             unsigned long long* __restrict__ counters) {
     __shared__ double s[6][kThreads];
+    for (int k = threadIdx.x; k < flag_words; k += kThreads) {
+        flags[k] = 0u;
+    }
+    if (threadIdx.x < 2) {
+        counters[threadIdx.x] = 0ull;
+    }
     double lo[3] = {partial[0], partial[1], partial[2]};
     double hi[3] = {partial[3], partial[4], partial[5]};
     for (int b = threadIdx.x; b < blocks; b += kThreads) {
@@ -1755,7 +1766,6 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error,
 
     double* root = B<double>(s.partial) + 6 * kBBoxBlocks;
     unsigned* flags = B<unsigned>(s.flags);
-    NBX_HIP(error, hipMemsetAsync(flags, 0, kFlagWords * sizeof(unsigned), st));
     if (fp32_inputs) {
         bbox_partial_kernel<true><<<bb_blocks, kThreads, 0, st>>>(e->px, e->py, e->pz, e->m, b4,
                                                                   ni, B<double>(s.partial));
@@ -1764,7 +1774,8 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error,
                                                                    ni, B<double>(s.partial));
     }
     bbox_final_kernel<<<1, kThreads, 0, st>>>(B<double>(s.partial), static_cast<int>(bb_blocks),
-                                              root);
+                                              root, flags, kFlagWords,
+                                              B<unsigned long long>(s.counters));
     keys_kernel<<<g, kThreads, 0, st>>>(b4, ni, root, B<uint64_t>(s.mkey),
                                         B<uint64_t>(s.tkey), B<uint64_t>(s.sk0),
                                         B<uint32_t>(s.sv0), flags);
@@ -1996,7 +2007,7 @@ NBodySimProStatus bh_walk(Engine* e, const NBodySimProGravity* g, bool fp64_exac
     BHState& s = e->bh;
     hipStream_t st = e->stream;
     const int ni = static_cast<int>(s.n);
-    NBX_HIP(error, hipMemsetAsync(s.counters.p, 0, 2 * sizeof(unsigned long long), st));
+    // the counters were cleared by this build's bbox_final_kernel
     NBX_HIP(error, hipEventRecord(e->ev[5], st));
     const double theta2 = e->theta * e->theta;
     const unsigned blocks = grid_for(s.n, kThreads);

@@ -177,6 +177,7 @@ NBodySimProStatus barnes_hut(Engine* e, const NBodySimProGravity* g, bool fp64_e
                              NBodySimProError* error);
 NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error,
                            bool quadrupoles = false);
+// One walk per bh_build: the build clears the walk's interaction counters.
 NBodySimProStatus bh_walk(Engine* e, const NBodySimProGravity* g, bool fp64_exact,
                           float* features, NBodySimProError* error);
 NBodySimProStatus bh_learned(Engine* e, const NBodySimProGravity* g, NBodySimProError* error);
