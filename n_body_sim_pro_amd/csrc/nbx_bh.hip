@@ -786,7 +786,7 @@ __device__ inline void com_node(int L, int depth, GroupNode* __restrict__ gnode,
                                 const double4* __restrict__ sp4, const double4* __restrict__ b4,
                                 double root_h, NodeF64* __restrict__ n64,
                                 NodeF32* __restrict__ n32, double* __restrict__ im2,
-                                float4* __restrict__ gquad, float4* __restrict__ gquad_pre) {
+                                float4* __restrict__ gquad) {
     const int4 r = in.rec[L];
     const int p = r.x;
     const int4 k0 = reinterpret_cast<const int4*>(gnode + L)[2];
@@ -896,9 +896,6 @@ __device__ inline void com_node(int L, int depth, GroupNode* __restrict__ gnode,
                                    static_cast<float>(3.0 * zz - tr), static_cast<float>(3.0 * xy));
         gquad[2 * L + 1] = make_float4(static_cast<float>(3.0 * xz), static_cast<float>(3.0 * yz),
                                        b.size2, b.inv_size2);
-        // the same record by pre-order node index, for the per-query walks
-        gquad_pre[2 * p] = gquad[2 * L];
-        gquad_pre[2 * p + 1] = gquad[2 * L + 1];
     }
     float4* g = reinterpret_cast<float4*>(gnode + L);
     g[0] = make_float4(b.cx, b.cy, b.cz, b.m);
@@ -911,12 +908,11 @@ __global__ __launch_bounds__(kThreads) void com_level_kernel(
     int first, int count, int depth, GroupNode* __restrict__ gnode,
     InternalSoA in, const uint32_t* __restrict__ trank, const double4* __restrict__ sp4,
     const double4* __restrict__ b4, const double* __restrict__ root, NodeF64* __restrict__ n64,
-    NodeF32* __restrict__ n32, double* __restrict__ im2, float4* __restrict__ gquad,
-    float4* __restrict__ gquad_pre) {
+    NodeF32* __restrict__ n32, double* __restrict__ im2, float4* __restrict__ gquad) {
     const int k = blockIdx.x * kThreads + threadIdx.x;
     if (k < count) {
         com_node<QUAD>(first + k, depth, gnode, in, trank, sp4, b4, root[3], n64, n32, im2,
-                       gquad, gquad_pre);
+                       gquad);
     }
 }
 
@@ -927,8 +923,7 @@ __global__ __launch_bounds__(1024) void com_small_levels_kernel(
     const unsigned* __restrict__ dcount, int d_hi, int d_lo, GroupNode* __restrict__ gnode,
     InternalSoA in, const uint32_t* __restrict__ trank, const double4* __restrict__ sp4,
     const double4* __restrict__ b4, const double* __restrict__ root, NodeF64* __restrict__ n64,
-    NodeF32* __restrict__ n32, double* __restrict__ im2, float4* __restrict__ gquad,
-    float4* __restrict__ gquad_pre) {
+    NodeF32* __restrict__ n32, double* __restrict__ im2, float4* __restrict__ gquad) {
     __shared__ unsigned doff[kMaxLevels];
     depth_offsets(dcount, doff);
     __syncthreads();
@@ -937,8 +932,7 @@ __global__ __launch_bounds__(1024) void com_small_levels_kernel(
         const int count = static_cast<int>(dcount[d]);
         const int first = static_cast<int>(doff[d]);
         for (int k = threadIdx.x; k < count; k += blockDim.x) {
-            com_node<QUAD>(first + k, d, gnode, in, trank, sp4, b4, root_h, n64, n32, im2, gquad,
-                           gquad_pre);
+            com_node<QUAD>(first + k, d, gnode, in, trank, sp4, b4, root_h, n64, n32, im2, gquad);
         }
         __syncthreads();
     }
@@ -1184,13 +1178,27 @@ __device__ inline bool walk_visit(const NodeF32& cur, int i, int t, float qx, fl
 // skip_until and its reference visit order. (Measured alternatives, slower on
 // MI355X: branch-free steps, vector-loaded nodes, prefetching both
 // successors, four queries per lane.)
+// The quadrupole record of pre-order node `cur` for the per-query walks: an
+// internal node carries its level index as leaf = -(L + 2); a leaf has none.
+__device__ inline void node_quad(const NodeF32& cur, const float4* __restrict__ gquad,
+                                 float4& q0, float4& q1) {
+    const int L = -cur.leaf - 2;
+    if (L >= 0) {
+        q0 = gquad[2 * L];
+        q1 = gquad[2 * L + 1];
+    } else {
+        q0 = make_float4(0.f, 0.f, 0.f, 0.f);
+        q1 = q0;
+    }
+}
+
 template <bool FEATURES, int Q>
 __global__ __launch_bounds__(kThreads) void walk_f32_kernel(
     const NodeF32* __restrict__ n32, int nodes, const float4* __restrict__ s4,
     const uint32_t* __restrict__ tperm, int n, float G, float eps2, float theta2,
     double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az,
     unsigned long long* __restrict__ counters, float* __restrict__ features,
-    const float4* __restrict__ pquad) {
+    const float4* __restrict__ gquad) {
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = xcd_block(blockIdx.x, gridDim.x) * (kThreads / kWave) + threadIdx.x / kWave;
     const int base = wave * (kWave * Q);
@@ -1211,9 +1219,11 @@ __global__ __launch_bounds__(kThreads) void walk_f32_kernel(
     while (i < nodes) {
         const int iu = __builtin_amdgcn_readfirstlane(i);
         const NodeF32 cur = n32[iu];
-        // with FEATURES the node's quadrupole record, fetched with the node
-        const float4 q0 = FEATURES ? pquad[2 * iu] : make_float4(0.f, 0.f, 0.f, 0.f);
-        const float4 q1 = FEATURES ? pquad[2 * iu + 1] : make_float4(0.f, 0.f, 0.f, 0.f);
+        // with FEATURES the node's quadrupole record
+        float4 q0 = make_float4(0.f, 0.f, 0.f, 0.f), q1 = q0;
+        if (FEATURES) {
+            node_quad(cur, gquad, q0, q1);
+        }
         bool descend = false;
 #pragma unroll
         for (int k = 0; k < Q; ++k) {
@@ -1330,7 +1340,7 @@ __global__ __launch_bounds__(kGroupThreads) void group_walk_kernel(
     double* __restrict__ ax, double* __restrict__ ay,
     double* __restrict__ az, unsigned long long* __restrict__ counters,
     float* __restrict__ features, const float4* __restrict__ gquad,
-    const float4* __restrict__ pquad, int max_walks, bool member_test) {
+    int max_walks, bool member_test) {
     constexpr int kWaves = kGroupThreads / kWave;
     // per wave: pending cells; staged cells (COM, mass; with FEATURES the
     // quadrupole record); staged leaves (tree position, then position and
@@ -1663,8 +1673,10 @@ __global__ __launch_bounds__(kGroupThreads) void group_walk_kernel(
         while (i < nodes) {
             const int iu = __builtin_amdgcn_readfirstlane(i);
             const NodeF32 cur = n32[iu];
-            const float4 q0 = FEATURES ? pquad[2 * iu] : make_float4(0.f, 0.f, 0.f, 0.f);
-            const float4 q1 = FEATURES ? pquad[2 * iu + 1] : make_float4(0.f, 0.f, 0.f, 0.f);
+            float4 q0 = make_float4(0.f, 0.f, 0.f, 0.f), q1 = q0;
+            if (FEATURES) {
+                node_quad(cur, gquad, q0, q1);
+            }
             const bool descend = walk_visit<FEATURES>(cur, i, t, qx, qy, qz, eps2, theta2,
                                                       skip_until, acc, q0, q1);
             i = cur.leaf >= 0 || __builtin_amdgcn_ballot_w64(descend) != 0 ? i + 1 : cur.skip;
@@ -1911,15 +1923,12 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error,
     // quadrupole records only for the learned correction's feature walk
     double* im2 = nullptr;
     float4* gquad = nullptr;
-    float4* gquad_pre = nullptr;
     s.has_quad = false;
     if (quadrupoles) {
         NBX_TRY(s.im2.need<double>(6 * (internal + 1), error));
         NBX_TRY(s.gquad.need<float4>(2 * (internal + 1), error));
-        NBX_TRY(s.gquad_pre.need<float4>(2 * nodes, error));
         im2 = B<double>(s.im2);
         gquad = B<float4>(s.gquad);
-        gquad_pre = B<float4>(s.gquad_pre);
         s.has_quad = true;
     }
     if (!fp32_inputs) {
@@ -1959,12 +1968,12 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error,
                 com_level_kernel<true><<<grid_for(count, kThreads), kThreads, 0, st>>>(
                     static_cast<int>(offs[d]), static_cast<int>(count), d,
                     B<GroupNode>(s.gnode), in, B<uint32_t>(s.trank), sp4, b4, root, n64_out,
-                    B<NodeF32>(s.n32), im2, gquad, gquad_pre);
+                    B<NodeF32>(s.n32), im2, gquad);
             } else {
                 com_level_kernel<false><<<grid_for(count, kThreads), kThreads, 0, st>>>(
                     static_cast<int>(offs[d]), static_cast<int>(count), d,
                     B<GroupNode>(s.gnode), in, B<uint32_t>(s.trank), sp4, b4, root, n64_out,
-                    B<NodeF32>(s.n32), nullptr, nullptr, nullptr);
+                    B<NodeF32>(s.n32), nullptr, nullptr);
             }
             --d;
             continue;
@@ -1976,13 +1985,12 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error,
         if (quadrupoles) {
             com_small_levels_kernel<true><<<1, 1024, 0, st>>>(
                 flags + kDCount, d, lo, B<GroupNode>(s.gnode), in,
-                B<uint32_t>(s.trank), sp4, b4, root, n64_out, B<NodeF32>(s.n32), im2, gquad,
-                gquad_pre);
+                B<uint32_t>(s.trank), sp4, b4, root, n64_out, B<NodeF32>(s.n32), im2, gquad);
         } else {
             com_small_levels_kernel<false><<<1, 1024, 0, st>>>(
                 flags + kDCount, d, lo, B<GroupNode>(s.gnode), in,
                 B<uint32_t>(s.trank), sp4, b4, root, n64_out, B<NodeF32>(s.n32), nullptr,
-                nullptr, nullptr);
+                nullptr);
         }
         d = lo - 1;
     }
@@ -2021,7 +2029,6 @@ NBodySimProStatus bh_walk(Engine* e, const NBodySimProGravity* g, bool fp64_exac
                  "the feature walk needs a build with quadrupole records");
     }
     const float4* gq = features != nullptr ? B<float4>(s.gquad) : nullptr;
-    const float4* gqp = features != nullptr ? B<float4>(s.gquad_pre) : nullptr;
     if (fp64_exact) {
         walk_f64_kernel<<<blocks, kThreads, 0, st>>>(
             B<NodeF64>(s.n64), static_cast<int>(s.nodes), B<double4>(s.sp4), B<uint32_t>(s.tperm), ni, g->gravitational_constant,
@@ -2053,12 +2060,12 @@ NBodySimProStatus bh_walk(Engine* e, const NBodySimProGravity* g, bool fp64_exac
             group_walk_kernel<true><<<grid_q(1), kGroupThreads, 0, st>>>(
                 nodes32, nn, gn, ni_cells, B<float4>(s.s4), B<uint32_t>(s.tperm),
                 B<uint64_t>(s.tword), s.ext_tree, ni, Gf, e2, th2, e->ax, e->ay, e->az, cnts,
-                features, gq, gqp, max_walks, member_test);
+                features, gq, max_walks, member_test);
         } else {
             group_walk_kernel<false><<<grid_q(1), kGroupThreads, 0, st>>>(
                 nodes32, nn, gn, ni_cells, B<float4>(s.s4), B<uint32_t>(s.tperm),
                 B<uint64_t>(s.tword), s.ext_tree, ni, Gf, e2, th2, e->ax, e->ay, e->az, cnts,
-                nullptr, nullptr, nullptr, max_walks, member_test);
+                nullptr, nullptr, max_walks, member_test);
         }
     } else {
         const unsigned wblocks = grid_for(s.n, kThreads * kWalkQueries);
@@ -2068,7 +2075,7 @@ NBodySimProStatus bh_walk(Engine* e, const NBodySimProGravity* g, bool fp64_exac
                 B<NodeF32>(s.n32), static_cast<int>(s.nodes), B<float4>(s.s4),
                 B<uint32_t>(s.tperm), ni, static_cast<float>(g->gravitational_constant),
                 static_cast<float>(g->softening_squared), static_cast<float>(theta2), e->ax,
-                e->ay, e->az, B<unsigned long long>(s.counters), features, gqp);
+                e->ay, e->az, B<unsigned long long>(s.counters), features, gq);
         } else {
             walk_f32_kernel<false, kWalkQueries><<<wgrid, kThreads, 0, st>>>(
                 B<NodeF32>(s.n32), static_cast<int>(s.nodes), B<float4>(s.s4),

@@ -72,7 +72,7 @@ auto work_buffers(Engine* e) {
             &s.hist, &s.mperm, &s.mkey_sorted, &s.tperm, &s.trank, &s.tword, &s.ext, &s.delta,
             &s.cnt, &s.scan, &s.scan_part, &s.sp4, &s.s4, &s.tile_hist, &s.tile_base, &s.n64,
             &s.n32, &s.icom, &s.irec, &s.idepth, &s.gnode, &s.im2, &s.gquad,
-            &s.gquad_pre, &s.tmp32, &s.counters, &e->scratch, &e->scratch2};
+            &s.tmp32, &s.counters, &e->scratch, &e->scratch2};
 }
 
 void free_buf(Buf& b) {

@@ -136,9 +136,10 @@ struct BHState {
     // table)
     Buf icom, irec, idepth, gnode;
     // second moments about the centre of mass (FP64, 6 per cell) and the
-    // walk's quadrupole records (2 float4 per cell, by level index and by
-    // pre-order node index), learned path only
-    Buf im2, gquad, gquad_pre;
+    // walk's quadrupole records (2 float4 per cell, by level index; the
+    // per-query walks reach them through the level index in their node
+    // records), learned path only
+    Buf im2, gquad;
     Buf tmp32;                // scratch permutation
     Buf counters;             // unsigned long long [approx, exact]
 };
