@@ -1355,6 +1355,7 @@ __global__ __launch_bounds__(kGroupThreads) void group_walk_kernel(
     // FEATURES the two quadrupole float4), so one LDS pointer walks the list.
     constexpr int kList = FEATURES ? kGroupList / 2 : kGroupList;
     static_assert(kList >= kWave, "a step stages up to one accepted cell per lane");
+    static_assert(kGroupLeaves >= kWave, "the member test stages a step's candidates in lleaf");
     // The plain walk evaluates its staged lists four entries per loop trip
     // (LDS loads run ahead, one address update per four; same summation
     // order). Measured on MI355X against one per trip: walk 7% faster at 1M
