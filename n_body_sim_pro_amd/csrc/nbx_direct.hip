@@ -30,11 +30,14 @@ namespace {
 constexpr int kExactTargets = 64;
 constexpr int kExactSources = 16;
 
-// One tile of terms: kExactSources sources x the block's 64 targets, each of
-// the block's WAVES waves forming kExactSources / WAVES sources' terms with
-// the reference's operations.
-template <int WAVES>
+// One tile of terms: kExactSources sources (from j0) x the block's 64
+// targets, each of the block's WAVES waves forming kExactSources / WAVES
+// sources' terms with the reference's operations. STAGE: the tile's sources
+// (x, y, z, m) are read from the LDS stage `src`; otherwise from memory (the
+// last source repeated past n).
+template <int WAVES, bool STAGE>
 __device__ inline void exact_terms(double (*terms)[kExactSources][kExactTargets],
+                                   const double (*src)[kExactSources],
                                    const double* __restrict__ px, const double* __restrict__ py,
                                    const double* __restrict__ pz, const double* __restrict__ m,
                                    int n, int j0, int wave, int lane, double xi, double yi,
@@ -44,19 +47,45 @@ __device__ inline void exact_terms(double (*terms)[kExactSources][kExactTargets]
 #pragma unroll
     for (int k = 0; k < kPerWave; ++k) {
         const int jj = wave * kPerWave + k;
-        const int j = j0 + jj;
-        const int jc = j < n ? j : n - 1;
-        const double dx = px[jc] - xi;
-        const double dy = py[jc] - yi;
-        const double dz = pz[jc] - zi;
+        double sx, sy, sz, sm;
+        if (STAGE) {
+            sx = src[0][jj];
+            sy = src[1][jj];
+            sz = src[2][jj];
+            sm = src[3][jj];
+        } else {
+            const int j = j0 + jj;
+            const int jc = j < n ? j : n - 1;
+            sx = px[jc];
+            sy = py[jc];
+            sz = pz[jc];
+            sm = m[jc];
+        }
+        const double dx = sx - xi;
+        const double dy = sy - yi;
+        const double dz = sz - zi;
         const double r2 = dx * dx + dy * dy + dz * dz + eps2;
         const double inv = 1.0 / sqrt(r2);
         const double inv3 = inv * inv * inv;
-        const double s = G * m[jc] * inv3;
+        const double s = G * sm * inv3;
         terms[0][jj][lane] = s * dx;
         terms[1][jj][lane] = s * dy;
         terms[2][jj][lane] = s * dz;
     }
+}
+
+// Loads (x, y, z, m) of the tile's kExactSources sources starting at j0 (the
+// last source repeated past n), one value per lane of the calling wave:
+// lane = field * kExactSources + source.
+static_assert(4 * kExactSources == kWave, "one staged source value per lane");
+__device__ inline double stage_value(const double* __restrict__ px, const double* __restrict__ py,
+                                     const double* __restrict__ pz, const double* __restrict__ m,
+                                     int n, int j0, int lane) {
+    const int f = lane / kExactSources;
+    const int j = j0 + lane % kExactSources;
+    const int jc = j < n ? j : n - 1;
+    const double* a = f == 0 ? px : (f == 1 ? py : (f == 2 ? pz : m));
+    return a[jc];
 }
 
 // SUBSET: the block's targets are targets[k] for k = blockIdx.x * 64 + lane < count,
@@ -73,13 +102,21 @@ __device__ inline void exact_terms(double (*terms)[kExactSources][kExactTargets]
 //   the earlier single-buffered 4-wave kernel: 1.69 / - / 192.
 // Loading a tile's 16 terms before its 16 adds (instead of one LDS wait per
 // add): 1.16 -> 0.82 / 11.9 -> 9.9 / 157 -> 136 ms with the dispatch above.
-template <bool SUBSET, int WAVES>
+// STAGE (from 32768 sources on): the sources come through an LDS stage that
+// the last wave fills one tile ahead (9.9 -> 9.4 ms at 65536, 136 -> 131 ms
+// at 262144, the 8192-target checker at 10M 0.57 -> 0.45 s; at 16384, where
+// the sources stay in cache, 0.82 -> 0.85 ms, so it is off there).
+template <bool SUBSET, int WAVES, bool STAGE>
 __global__ __launch_bounds__(WAVES * 64) void direct_fp64_kernel(
     const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
     const double* __restrict__ m, double* __restrict__ ax, double* __restrict__ ay,
     double* __restrict__ az, int n, double G, double eps2, const int64_t* __restrict__ targets,
     int count, double* __restrict__ out) {
     __shared__ double terms[2][3][kExactSources][kExactTargets];
+    // sources of the next two tiles: the last wave loads tile t + 2's while
+    // the block forms tile t + 1's terms, so no wave waits on a global load
+    // after a barrier
+    __shared__ double src[2][4][kExactSources];
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
     const int k = blockIdx.x * kExactTargets + lane;
@@ -89,7 +126,14 @@ __global__ __launch_bounds__(WAVES * 64) void direct_fp64_kernel(
     const double xi = px[ic], yi = py[ic], zi = pz[ic];
     double acc = 0.0;
     const int tiles = (n + kExactSources - 1) / kExactSources;
-    exact_terms<WAVES>(terms[0], px, py, pz, m, n, 0, wave, lane, xi, yi, zi, G, eps2);
+    constexpr int kLoader = WAVES - 1;
+    if (STAGE && wave == kLoader) {
+        (&src[0][0][0])[lane] = stage_value(px, py, pz, m, n, 0, lane);
+        (&src[1][0][0])[lane] = stage_value(px, py, pz, m, n, kExactSources, lane);
+    }
+    __syncthreads();
+    exact_terms<WAVES, STAGE>(terms[0], src[0], px, py, pz, m, n, 0, wave, lane, xi, yi, zi, G,
+                              eps2);
     __syncthreads();
     for (int t = 0; t < tiles; ++t) {
         const int j0 = t * kExactSources;
@@ -121,8 +165,16 @@ __global__ __launch_bounds__(WAVES * 64) void direct_fp64_kernel(
             }
         }
         if (t + 1 < tiles) {
-            exact_terms<WAVES>(terms[(t + 1) & 1], px, py, pz, m, n, j0 + kExactSources, wave,
-                               lane, xi, yi, zi, G, eps2);
+            // tile t + 2's sources go to the stage tile t used
+            double next = 0.0;
+            if (STAGE && wave == kLoader) {
+                next = stage_value(px, py, pz, m, n, j0 + 2 * kExactSources, lane);
+            }
+            exact_terms<WAVES, STAGE>(terms[(t + 1) & 1], src[(t + 1) & 1], px, py, pz, m, n,
+                                      j0 + kExactSources, wave, lane, xi, yi, zi, G, eps2);
+            if (STAGE && wave == kLoader) {
+                (&src[t & 1][0][0])[lane] = next;
+            }
         }
         __syncthreads();
     }
@@ -144,12 +196,23 @@ void launch_direct_fp64(hipStream_t st, int cu_count, const double* px, const do
                         int n, double G, double eps2, const int64_t* targets, int count,
                         double* out) {
     const unsigned blocks = grid_for(static_cast<size_t>(count), kExactTargets);
+    const bool stage = n >= 32768;
     if (blocks < 2u * static_cast<unsigned>(cu_count)) {
-        direct_fp64_kernel<SUBSET, 16><<<blocks, 16 * 64, 0, st>>>(px, py, pz, m, ax, ay, az, n,
-                                                                   G, eps2, targets, count, out);
+        if (stage) {
+            direct_fp64_kernel<SUBSET, 16, true><<<blocks, 16 * 64, 0, st>>>(
+                px, py, pz, m, ax, ay, az, n, G, eps2, targets, count, out);
+        } else {
+            direct_fp64_kernel<SUBSET, 16, false><<<blocks, 16 * 64, 0, st>>>(
+                px, py, pz, m, ax, ay, az, n, G, eps2, targets, count, out);
+        }
     } else {
-        direct_fp64_kernel<SUBSET, 8><<<blocks, 8 * 64, 0, st>>>(px, py, pz, m, ax, ay, az, n, G,
-                                                                 eps2, targets, count, out);
+        if (stage) {
+            direct_fp64_kernel<SUBSET, 8, true><<<blocks, 8 * 64, 0, st>>>(
+                px, py, pz, m, ax, ay, az, n, G, eps2, targets, count, out);
+        } else {
+            direct_fp64_kernel<SUBSET, 8, false><<<blocks, 8 * 64, 0, st>>>(
+                px, py, pz, m, ax, ay, az, n, G, eps2, targets, count, out);
+        }
     }
 }
 
