@@ -54,7 +54,13 @@ NBodySimProStatus ensure(void** ptr, size_t* cap, size_t bytes, NBodySimProError
         *ptr = nullptr;
         *cap = 0;
     }
-    size_t want = bytes < 256 ? 256 : bytes;
+    // One eighth of headroom: sizes that follow the tree (cells, nodes) move
+    // by a fraction of a percent from step to step, and every new maximum
+    // would otherwise cost a hipFree (which waits for the device) and a
+    // hipMalloc inside the evaluation. (Measured on MI355X, 1M two-galaxy
+    // leapfrog: one build in eight paid about 0.2 ms for it.)
+    size_t want = bytes + bytes / 8;
+    want = want < 256 ? 256 : want;
     NBX_HIP(error, hipMalloc(ptr, want));
     *cap = want;
     return N_BODY_SIM_PRO_STATUS_OK;
