@@ -247,8 +247,8 @@ __global__ __launch_bounds__(kThreads) void bbox_partial_kernel(
     }
 }
 
-// Also clears the build's flag words and the walk's interaction counters (no
-// fill launches of their own inside the evaluation).
+// Also clears the build's flag words and the walk's counters (interactions,
+// deferred queries; no fill launches of their own inside the evaluation).
 __global__ __launch_bounds__(kThreads) void bbox_final_kernel(const double* __restrict__ partial,
                                                              int blocks,
                                                              double* __restrict__ root,
@@ -259,7 +259,7 @@ __global__ __launch_bounds__(kThreads) void bbox_final_kernel(const double* __re
     for (int k = threadIdx.x; k < flag_words; k += kThreads) {
         flags[k] = 0u;
     }
-    if (threadIdx.x < 2) {
+    if (threadIdx.x < 3) {
         counters[threadIdx.x] = 0ull;
     }
     double lo[3] = {partial[0], partial[1], partial[2]};
@@ -692,8 +692,8 @@ __global__ __launch_bounds__(kTopoTile) void ends_kernel(const int32_t* __restri
     }
 }
 
-// Static fields of the internal cells (by level index), the walk records of
-// the leaves, and every node's slot in its parent's child table
+// Static fields of the internal cells (by level index), the FP64 walk records
+// of the leaves (FP64 builds), and every node's slot in its parent's child table
 // gnode[parent L].kids[octant] (internal child: its level index; leaf child:
 // -(tree position + 2)). A position's cells are visited shallow to deep, so each one's parent
 // is the cell visited just before it: the depth-delta[t-1] cell that holds
@@ -703,8 +703,8 @@ __global__ __launch_bounds__(kTopoTile) void fill_kernel(
     const int32_t* __restrict__ delta, const uint32_t* __restrict__ cnt,
     const uint32_t* __restrict__ scan, int n, int tiles, const double4* __restrict__ sp4,
     const double* __restrict__ root, InternalSoA in, NodeF64* __restrict__ n64,
-    NodeF32* __restrict__ n32, const unsigned* __restrict__ dcount,
-    const unsigned* __restrict__ start_base, GroupNode* __restrict__ gnode) {
+    const unsigned* __restrict__ dcount, const unsigned* __restrict__ start_base,
+    GroupNode* __restrict__ gnode) {
     __shared__ TileRanks r;
     tile_ranks_init(r, dcount, start_base, tiles);
     const double root_h = root[3];
@@ -745,12 +745,12 @@ __global__ __launch_bounds__(kTopoTile) void fill_kernel(
     }
     if (valid) {
         const int depth = 1 + (dprev > hi ? dprev : hi);
-        const int li = static_cast<int>(sc + cnt[t]) + t;
         if (depth > 0 && parent >= 0) {
             gnode[parent].kids[digit_at(w, ext, t, depth - 1)] = -(t + 2);
         }
-        const double4 q = sp4[t];
         if (n64 != nullptr) {   // FP64 records only for the exact FP64 walk
+            const int li = static_cast<int>(sc + cnt[t]) + t;
+            const double4 q = sp4[t];
             NodeF64 a;
             a.cx = q.x;
             a.cy = q.y;
@@ -761,17 +761,37 @@ __global__ __launch_bounds__(kTopoTile) void fill_kernel(
             a.leaf = t;
             n64[li] = a;
         }
-        NodeF32 b;
-        b.cx = static_cast<float>(q.x);
-        b.cy = static_cast<float>(q.y);
-        b.cz = static_cast<float>(q.z);
-        b.m = static_cast<float>(q.w);
-        b.size2 = 0.f;
-        b.skip = li + 1;
-        b.leaf = t;
-        b.inv_size2 = 0.f;
-        n32[li] = b;
     }
+}
+
+// The FP32 pre-order record of a leaf (tree position t, node index li) and of
+// an internal cell (level index L, its FP64 centre of mass and mass, half
+// size h, pre-order skip). Only the per-query walks read these records, so
+// node_records_kernel writes them on demand after the build.
+__device__ inline NodeF32 leaf_record(const double4& q, int li, int t) {
+    NodeF32 b;
+    b.cx = static_cast<float>(q.x);
+    b.cy = static_cast<float>(q.y);
+    b.cz = static_cast<float>(q.z);
+    b.m = static_cast<float>(q.w);
+    b.size2 = 0.f;
+    b.skip = li + 1;
+    b.leaf = t;
+    b.inv_size2 = 0.f;
+    return b;
+}
+
+__device__ inline NodeF32 cell_record(const double4& c, double h, int skip, int L) {
+    NodeF32 b;
+    b.cx = static_cast<float>(c.x);
+    b.cy = static_cast<float>(c.y);
+    b.cz = static_cast<float>(c.z);
+    b.m = static_cast<float>(c.w);
+    b.size2 = static_cast<float>(4.0 * h * h);
+    b.skip = skip;
+    b.leaf = -(L + 2);   // internal: -(level index + 2)
+    b.inv_size2 = 1.f / b.size2;
+    return b;
 }
 
 // One node of the centre-of-mass pass (barnes_hut.c:214-253): children
@@ -785,8 +805,7 @@ __device__ inline void com_node(int L, int depth, GroupNode* __restrict__ gnode,
                                 InternalSoA in, const uint32_t* __restrict__ trank,
                                 const double4* __restrict__ sp4, const double4* __restrict__ b4,
                                 double root_h, NodeF64* __restrict__ n64,
-                                NodeF32* __restrict__ n32, double* __restrict__ im2,
-                                float4* __restrict__ gquad) {
+                                double* __restrict__ im2, float4* __restrict__ gquad) {
     const int4 r = in.rec[L];
     const int p = r.x;
     const int4 k0 = reinterpret_cast<const int4*>(gnode + L)[2];
@@ -844,16 +863,7 @@ __device__ inline void com_node(int L, int depth, GroupNode* __restrict__ gnode,
         a.leaf = -1;
         n64[p] = a;
     }
-    NodeF32 b;
-    b.cx = static_cast<float>(ox);
-    b.cy = static_cast<float>(oy);
-    b.cz = static_cast<float>(oz);
-    b.m = static_cast<float>(mass);
-    b.size2 = static_cast<float>(4.0 * h * h);
-    b.skip = r.y;
-    b.leaf = -(L + 2);   // internal: -(level index + 2)
-    b.inv_size2 = 1.f / b.size2;
-    n32[p] = b;
+    const NodeF32 b = cell_record(make_double4(ox, oy, oz, mass), h, r.y, L);
     if (QUAD) {
         // second moment about the centre of mass from the children's
         // (parallel axes), FP64; the walk's traceless FP32 quadrupole
@@ -908,11 +918,10 @@ __global__ __launch_bounds__(kThreads) void com_level_kernel(
     int first, int count, int depth, GroupNode* __restrict__ gnode,
     InternalSoA in, const uint32_t* __restrict__ trank, const double4* __restrict__ sp4,
     const double4* __restrict__ b4, const double* __restrict__ root, NodeF64* __restrict__ n64,
-    NodeF32* __restrict__ n32, double* __restrict__ im2, float4* __restrict__ gquad) {
+    double* __restrict__ im2, float4* __restrict__ gquad) {
     const int k = blockIdx.x * kThreads + threadIdx.x;
     if (k < count) {
-        com_node<QUAD>(first + k, depth, gnode, in, trank, sp4, b4, root[3], n64, n32, im2,
-                       gquad);
+        com_node<QUAD>(first + k, depth, gnode, in, trank, sp4, b4, root[3], n64, im2, gquad);
     }
 }
 
@@ -923,7 +932,7 @@ __global__ __launch_bounds__(1024) void com_small_levels_kernel(
     const unsigned* __restrict__ dcount, int d_hi, int d_lo, GroupNode* __restrict__ gnode,
     InternalSoA in, const uint32_t* __restrict__ trank, const double4* __restrict__ sp4,
     const double4* __restrict__ b4, const double* __restrict__ root, NodeF64* __restrict__ n64,
-    NodeF32* __restrict__ n32, double* __restrict__ im2, float4* __restrict__ gquad) {
+    double* __restrict__ im2, float4* __restrict__ gquad) {
     __shared__ unsigned doff[kMaxLevels];
     depth_offsets(dcount, doff);
     __syncthreads();
@@ -932,9 +941,28 @@ __global__ __launch_bounds__(1024) void com_small_levels_kernel(
         const int count = static_cast<int>(dcount[d]);
         const int first = static_cast<int>(doff[d]);
         for (int k = threadIdx.x; k < count; k += blockDim.x) {
-            com_node<QUAD>(first + k, d, gnode, in, trank, sp4, b4, root_h, n64, n32, im2, gquad);
+            com_node<QUAD>(first + k, d, gnode, in, trank, sp4, b4, root_h, n64, im2, gquad);
         }
         __syncthreads();
+    }
+}
+
+// The FP32 pre-order records of every node, for the per-query walks, from a
+// finished build: thread k < n writes the leaf at tree position k (node index
+// scan[k] + cnt[k] + k), thread n + L the internal cell of level index L (node
+// index rec[L].x). The values are those the build's own FP64 records round to.
+__global__ __launch_bounds__(kThreads) void node_records_kernel(
+    int n, int internal, const uint32_t* __restrict__ scan, const uint32_t* __restrict__ cnt,
+    const double4* __restrict__ sp4, InternalSoA in, const double* __restrict__ root,
+    NodeF32* __restrict__ n32) {
+    const int k = blockIdx.x * kThreads + threadIdx.x;
+    if (k < n) {
+        const int li = static_cast<int>(scan[k] + cnt[k]) + k;
+        n32[li] = leaf_record(sp4[k], li, k);
+    } else if (k - n < internal) {
+        const int L = k - n;
+        const int4 r = in.rec[L];
+        n32[r.x] = cell_record(in.com[L], ldexp(root[3], -in.depth[L]), r.y, L);
     }
 }
 
@@ -1192,28 +1220,32 @@ __device__ inline void node_quad(const NodeF32& cur, const float4* __restrict__ 
     }
 }
 
+// Queries: tree positions 0 .. nq - 1, or with `list` the tree positions
+// list[0 .. nq - 1] (the queries a group walk deferred).
 template <bool FEATURES, int Q>
 __global__ __launch_bounds__(kThreads) void walk_f32_kernel(
     const NodeF32* __restrict__ n32, int nodes, const float4* __restrict__ s4,
-    const uint32_t* __restrict__ tperm, int n, float G, float eps2, float theta2,
-    double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az,
-    unsigned long long* __restrict__ counters, float* __restrict__ features,
-    const float4* __restrict__ gquad) {
+    const uint32_t* __restrict__ tperm, const int* __restrict__ list, int nq, float G,
+    float eps2, float theta2, double* __restrict__ ax, double* __restrict__ ay,
+    double* __restrict__ az, unsigned long long* __restrict__ counters,
+    float* __restrict__ features, const float4* __restrict__ gquad) {
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = xcd_block(blockIdx.x, gridDim.x) * (kThreads / kWave) + threadIdx.x / kWave;
     const int base = wave * (kWave * Q);
     int t[Q], skip_until[Q];
+    bool valid[Q];
     float qx[Q], qy[Q], qz[Q];
     WalkAcc<FEATURES> acc[Q];
 #pragma unroll
     for (int k = 0; k < Q; ++k) {
-        t[k] = base + k * kWave + lane;
-        const bool valid = t[k] < n;
-        const float4 q = s4[valid ? t[k] : n - 1];
+        const int qi = base + k * kWave + lane;
+        valid[k] = qi < nq;
+        t[k] = valid[k] ? (list != nullptr ? list[qi] : qi) : -1;
+        const float4 q = s4[valid[k] ? t[k] : 0];
         qx[k] = q.x;
         qy[k] = q.y;
         qz[k] = q.z;
-        skip_until[k] = valid ? 0 : 0x7FFFFFFF;
+        skip_until[k] = valid[k] ? 0 : 0x7FFFFFFF;
     }
     int i = 0;
     while (i < nodes) {
@@ -1235,7 +1267,7 @@ __global__ __launch_bounds__(kThreads) void walk_f32_kernel(
     unsigned long long wa = 0, we = 0;
 #pragma unroll
     for (int k = 0; k < Q; ++k) {
-        if (t[k] < n) {
+        if (valid[k]) {
             wa += acc[k].approx;
             we += acc[k].exact;
             acc[k].store(t[k], tperm, G, ax, ay, az, features);
@@ -1264,9 +1296,10 @@ __global__ __launch_bounds__(kThreads) void walk_f32_kernel(
 // cells and leaves are evaluated by all 64 lanes without divergence (leaf
 // positions are gathered into LDS just before). A group that is not compact
 // (64 consecutive positions of the Morton curve can straddle one of its
-// jumps) would get a box with most of the tree in its near field; such a wave
-// walks per query with reference lists instead, as does a wave whose stack
-// would overflow.
+// jumps) would get a box with most of the tree in its near field: it is split
+// into compact parts. The members of a part the split cannot reach within its
+// budget, or whose stack would overflow, are deferred to the per-query walk
+// with reference lists, which runs after this kernel.
 constexpr int kGroupList = 128;      // staged cells per wave
 // 96 staged leaves: less LDS per wave, so more waves per CU (measured on
 // MI355X against 128: 3% faster walk at 1M particles, 6% at 10M)
@@ -1333,7 +1366,7 @@ __device__ inline void wave_lds_fence() {
 
 template <bool FEATURES>
 __global__ __launch_bounds__(kGroupThreads) void group_walk_kernel(
-    const NodeF32* __restrict__ n32, int nodes, const GroupNode* __restrict__ gnode,
+    int* __restrict__ deferred, const GroupNode* __restrict__ gnode,
     int internal, const float4* __restrict__ s4,
     const uint32_t* __restrict__ tperm, const uint64_t* __restrict__ tword,
     const uint64_t* __restrict__ ext, int n, float G, float eps2, float theta2,
@@ -1610,9 +1643,9 @@ __global__ __launch_bounds__(kGroupThreads) void group_walk_kernel(
     // (a range straddling a jump, an outlier, a dense core and its outskirts
     // would otherwise get a box with most of the tree in its near field). A
     // range that is not compact is split at the octant changes of its common
-    // cell's children (tree order is key order, so each octant is a run); a
-    // range that cannot be split within the budget of `max_walks` shared
-    // walks per wave walks per query.
+    // cell's children (tree order is key order, so each octant is a run); the
+    // members of a range that cannot be split within the budget of
+    // `max_walks` shared walks per wave are deferred to the per-query walk.
     const int last = t0 + kWave - 1 < n ? t0 + kWave - 1 : n - 1;
     const int max_level = ext != nullptr ? kMaxLevels : 21;
     int rsp = 1, walks = 0;
@@ -1667,24 +1700,24 @@ __global__ __launch_bounds__(kGroupThreads) void group_walk_kernel(
             per_query = per_query || (valid && t >= a && t <= b);
         }
     }
-    if (__ballot(per_query) != 0) {
-        // reference lists for the lanes whose range could not share a list
-        int skip_until = per_query ? 0 : 0x7FFFFFFF;
-        int i = 0;
-        while (i < nodes) {
-            const int iu = __builtin_amdgcn_readfirstlane(i);
-            const NodeF32 cur = n32[iu];
-            float4 q0 = make_float4(0.f, 0.f, 0.f, 0.f), q1 = q0;
-            if (FEATURES) {
-                node_quad(cur, gquad, q0, q1);
-            }
-            const bool descend = walk_visit<FEATURES>(cur, i, t, qx, qy, qz, eps2, theta2,
-                                                      skip_until, acc, q0, q1);
-            i = cur.leaf >= 0 || __builtin_amdgcn_ballot_w64(descend) != 0 ? i + 1 : cur.skip;
+    // The lanes whose range could not share a list go to the deferred list
+    // (counters[2] entries): the per-query reference-list walk takes them after
+    // this kernel, and writes their results. Their sums here are empty (never
+    // walked, or cleared on overflow).
+    const uint64_t deferring = __builtin_amdgcn_ballot_w64(per_query);
+    if (deferring != 0) {
+        unsigned long long slot = 0;
+        if (lane == 0) {
+            slot = atomicAdd(&counters[2], static_cast<unsigned long long>(__popcll(deferring)));
+        }
+        slot = __shfl(slot, 0);
+        const unsigned long long k = slot + lanes_below(deferring);
+        if (per_query && k < static_cast<unsigned long long>(n)) {
+            deferred[k] = t;
         }
     }
     unsigned long long wa = 0, we = 0;
-    if (valid) {
+    if (valid && !per_query) {
         wa = acc.approx;
         we = acc.exact;
         acc.store(t, tperm, G, ax, ay, az, features);
@@ -1738,6 +1771,8 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error,
                            bool quadrupoles) {
     BHState& s = e->bh;
     const size_t n = e->n;
+    // the FP32 pre-order records are written on demand (bh_node_records)
+    s.has_n32 = false;
     if (n == 0) {
         NBX_FAIL(error, N_BODY_SIM_PRO_STATUS_INVALID_ARGUMENT,
                  "cannot build a Barnes-Hut tree for an empty system");
@@ -1772,7 +1807,8 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error,
     NBX_TRY(s.scan.need<uint32_t>(n + 1, error));
     NBX_TRY(s.sp4.need<double4>(n, error));
     NBX_TRY(s.s4.need<float4>(n, error));
-    NBX_TRY(s.counters.need<unsigned long long>(2, error));
+    NBX_TRY(s.counters.need<unsigned long long>(3, error));
+    NBX_TRY(s.deferred.need<int32_t>(n, error));
     const size_t tiles = (n + kTopoTile - 1) / kTopoTile;
     NBX_TRY(s.tile_hist.need<unsigned>(2 * kMaxLevels * tiles, error));
     NBX_TRY(s.tile_base.need<unsigned>(2 * kMaxLevels * tiles, error));
@@ -1937,7 +1973,6 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error,
         n64_out = B<NodeF64>(s.n64);
         s.has_n64 = true;
     }
-    NBX_TRY(s.n32.need<NodeF32>(nodes, error));
     const InternalSoA in = internal_soa(s);
     const double4* sp4 = B<double4>(s.sp4);
     s.ext_tree = ext_tree;
@@ -1948,7 +1983,7 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error,
                                          in.rec, B<GroupNode>(s.gnode));
     fill_kernel<<<nt, kTopoTile, 0, st>>>(B<uint64_t>(s.tword), ext_tree, B<int32_t>(s.delta),
                                          B<uint32_t>(s.cnt), B<uint32_t>(s.scan), ni, nt, sp4,
-                                         root, in, n64_out, B<NodeF32>(s.n32), flags + kDCount,
+                                         root, in, n64_out, flags + kDCount,
                                          start_base, B<GroupNode>(s.gnode));
     NBX_HIP(error, hipGetLastError());
     NBX_HIP(error, hipEventRecord(e->ev[3], st));
@@ -1969,12 +2004,12 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error,
                 com_level_kernel<true><<<grid_for(count, kThreads), kThreads, 0, st>>>(
                     static_cast<int>(offs[d]), static_cast<int>(count), d,
                     B<GroupNode>(s.gnode), in, B<uint32_t>(s.trank), sp4, b4, root, n64_out,
-                    B<NodeF32>(s.n32), im2, gquad);
+                    im2, gquad);
             } else {
                 com_level_kernel<false><<<grid_for(count, kThreads), kThreads, 0, st>>>(
                     static_cast<int>(offs[d]), static_cast<int>(count), d,
                     B<GroupNode>(s.gnode), in, B<uint32_t>(s.trank), sp4, b4, root, n64_out,
-                    B<NodeF32>(s.n32), nullptr, nullptr);
+                    nullptr, nullptr);
             }
             --d;
             continue;
@@ -1986,11 +2021,11 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error,
         if (quadrupoles) {
             com_small_levels_kernel<true><<<1, 1024, 0, st>>>(
                 flags + kDCount, d, lo, B<GroupNode>(s.gnode), in,
-                B<uint32_t>(s.trank), sp4, b4, root, n64_out, B<NodeF32>(s.n32), im2, gquad);
+                B<uint32_t>(s.trank), sp4, b4, root, n64_out, im2, gquad);
         } else {
             com_small_levels_kernel<false><<<1, 1024, 0, st>>>(
                 flags + kDCount, d, lo, B<GroupNode>(s.gnode), in,
-                B<uint32_t>(s.trank), sp4, b4, root, n64_out, B<NodeF32>(s.n32), nullptr,
+                B<uint32_t>(s.trank), sp4, b4, root, n64_out, nullptr,
                 nullptr);
         }
         d = lo - 1;
@@ -2012,6 +2047,25 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error,
     return N_BODY_SIM_PRO_STATUS_OK;
 }
 
+NBodySimProStatus bh_node_records(Engine* e, NBodySimProError* error) {
+    BHState& s = e->bh;
+    if (s.n == 0) {
+        NBX_FAIL(error, N_BODY_SIM_PRO_STATUS_INVALID_STATE, "no Barnes-Hut build");
+    }
+    if (s.has_n32) {
+        return N_BODY_SIM_PRO_STATUS_OK;
+    }
+    NBX_TRY(s.n32.need<NodeF32>(s.nodes, error));
+    const int ni = static_cast<int>(s.n);
+    const int ic = static_cast<int>(s.internal);
+    node_records_kernel<<<grid_for(s.nodes, kThreads), kThreads, 0, e->stream>>>(
+        ni, ic, B<uint32_t>(s.scan), B<uint32_t>(s.cnt), B<double4>(s.sp4), internal_soa(s),
+        B<double>(s.partial) + 6 * kBBoxBlocks, B<NodeF32>(s.n32));
+    NBX_HIP(error, hipGetLastError());
+    s.has_n32 = true;
+    return N_BODY_SIM_PRO_STATUS_OK;
+}
+
 NBodySimProStatus bh_walk(Engine* e, const NBodySimProGravity* g, bool fp64_exact, float* features,
                           NBodySimProError* error) {
     BHState& s = e->bh;
@@ -2030,19 +2084,40 @@ NBodySimProStatus bh_walk(Engine* e, const NBodySimProGravity* g, bool fp64_exac
                  "the feature walk needs a build with quadrupole records");
     }
     const float4* gq = features != nullptr ? B<float4>(s.gquad) : nullptr;
+    const float Gf = static_cast<float>(g->gravitational_constant);
+    const float e2 = static_cast<float>(g->softening_squared);
+    const float th2 = static_cast<float>(theta2);
+    unsigned long long* cnts = B<unsigned long long>(s.counters);
+    // per-query reference-list walk of nq queries (all tree positions, or the
+    // listed ones) on the FP32 pre-order records
+    auto per_query_walk = [&](const int* list, int nq) -> NBodySimProStatus {
+        NBX_TRY(bh_node_records(e, error));
+        const unsigned wblocks = grid_for(static_cast<size_t>(nq), kThreads * kWalkQueries);
+        const unsigned wgrid = (wblocks + kXcds - 1) / kXcds * kXcds;
+        const NodeF32* n32 = B<NodeF32>(s.n32);
+        const int nn = static_cast<int>(s.nodes);
+        if (features != nullptr) {
+            walk_f32_kernel<true, kWalkQueries><<<wgrid, kThreads, 0, st>>>(
+                n32, nn, B<float4>(s.s4), B<uint32_t>(s.tperm), list, nq, Gf, e2, th2, e->ax,
+                e->ay, e->az, cnts, features, gq);
+        } else {
+            walk_f32_kernel<false, kWalkQueries><<<wgrid, kThreads, 0, st>>>(
+                n32, nn, B<float4>(s.s4), B<uint32_t>(s.tperm), list, nq, Gf, e2, th2, e->ax,
+                e->ay, e->az, cnts, nullptr, nullptr);
+        }
+        NBX_HIP(error, hipGetLastError());
+        return N_BODY_SIM_PRO_STATUS_OK;
+    };
+    bool group = false;
     if (fp64_exact) {
         walk_f64_kernel<<<blocks, kThreads, 0, st>>>(
             B<NodeF64>(s.n64), static_cast<int>(s.nodes), B<double4>(s.sp4), B<uint32_t>(s.tperm), ni, g->gravitational_constant,
-            g->softening_squared, theta2, e->ax, e->ay, e->az, B<unsigned long long>(s.counters));
+            g->softening_squared, theta2, e->ax, e->ay, e->az, cnts);
     } else if (!(e->flags & NBX_FLAG_BH_REFERENCE_LISTS)) {
-        const NodeF32* nodes32 = B<NodeF32>(s.n32);
-        const int nn = static_cast<int>(s.nodes);
+        group = true;
         const GroupNode* gn = B<GroupNode>(s.gnode);
         const int ni_cells = static_cast<int>(s.internal);
-        const float Gf = static_cast<float>(g->gravitational_constant);
-        const float e2 = static_cast<float>(g->softening_squared);
-        const float th2 = static_cast<float>(theta2);
-        unsigned long long* cnts = B<unsigned long long>(s.counters);
+        int* deferred = B<int32_t>(s.deferred);
         // one shared walk per group (no split) or the split's budget
         const int max_walks = (e->flags & NBX_FLAG_BH_NO_SPLIT) ? 1 : kMaxSubWalks;
         // The member test is on unless the box test alone is asked for; the
@@ -2053,43 +2128,43 @@ NBodySimProStatus bh_walk(Engine* e, const NBodySimProGravity* g, bool fp64_exac
         // theta = 1.0, 10M: 9.61 -> 7.96 ms.)
         const bool member_test =
             features != nullptr || (e->flags & NBX_FLAG_BH_BOX_TEST) == 0;
-        auto grid_q = [&](int q) {
-            const unsigned blocks_q = grid_for(s.n, kGroupThreads * q);
-            return (blocks_q + kXcds - 1) / kXcds * kXcds;
-        };
+        const unsigned blocks_g = grid_for(s.n, kGroupThreads);
+        const unsigned grid_g = (blocks_g + kXcds - 1) / kXcds * kXcds;
         if (features != nullptr) {
-            group_walk_kernel<true><<<grid_q(1), kGroupThreads, 0, st>>>(
-                nodes32, nn, gn, ni_cells, B<float4>(s.s4), B<uint32_t>(s.tperm),
+            group_walk_kernel<true><<<grid_g, kGroupThreads, 0, st>>>(
+                deferred, gn, ni_cells, B<float4>(s.s4), B<uint32_t>(s.tperm),
                 B<uint64_t>(s.tword), s.ext_tree, ni, Gf, e2, th2, e->ax, e->ay, e->az, cnts,
                 features, gq, max_walks, member_test);
         } else {
-            group_walk_kernel<false><<<grid_q(1), kGroupThreads, 0, st>>>(
-                nodes32, nn, gn, ni_cells, B<float4>(s.s4), B<uint32_t>(s.tperm),
+            group_walk_kernel<false><<<grid_g, kGroupThreads, 0, st>>>(
+                deferred, gn, ni_cells, B<float4>(s.s4), B<uint32_t>(s.tperm),
                 B<uint64_t>(s.tword), s.ext_tree, ni, Gf, e2, th2, e->ax, e->ay, e->az, cnts,
                 nullptr, nullptr, max_walks, member_test);
         }
+        NBX_HIP(error, hipGetLastError());
     } else {
-        const unsigned wblocks = grid_for(s.n, kThreads * kWalkQueries);
-        const unsigned wgrid = (wblocks + kXcds - 1) / kXcds * kXcds;
-        if (features != nullptr) {
-            walk_f32_kernel<true, kWalkQueries><<<wgrid, kThreads, 0, st>>>(
-                B<NodeF32>(s.n32), static_cast<int>(s.nodes), B<float4>(s.s4),
-                B<uint32_t>(s.tperm), ni, static_cast<float>(g->gravitational_constant),
-                static_cast<float>(g->softening_squared), static_cast<float>(theta2), e->ax,
-                e->ay, e->az, B<unsigned long long>(s.counters), features, gq);
-        } else {
-            walk_f32_kernel<false, kWalkQueries><<<wgrid, kThreads, 0, st>>>(
-                B<NodeF32>(s.n32), static_cast<int>(s.nodes), B<float4>(s.s4),
-                B<uint32_t>(s.tperm), ni, static_cast<float>(g->gravitational_constant),
-                static_cast<float>(g->softening_squared), static_cast<float>(theta2), e->ax,
-                e->ay, e->az, B<unsigned long long>(s.counters), nullptr, nullptr);
-        }
+        NBX_TRY(per_query_walk(nullptr, ni));
     }
     NBX_HIP(error, hipGetLastError());
     NBX_HIP(error, hipEventRecord(e->ev[6], st));
-    unsigned long long counts[2] = {0, 0};
+    unsigned long long counts[3] = {0, 0, 0};
     NBX_HIP(error, hipMemcpyAsync(counts, s.counters.p, sizeof(counts), hipMemcpyDeviceToHost, st));
     NBX_HIP(error, hipStreamSynchronize(st));
+    if (group && counts[2] > 0) {
+        // Queries the group walk could not give a shared list (a range the
+        // split could not make compact within its budget, or a stack
+        // overflow; none in most evaluations): their reference lists, after
+        // the FP32 pre-order records are written.
+        if (counts[2] > s.n) {
+            NBX_FAIL(error, N_BODY_SIM_PRO_STATUS_INVALID_STATE,
+                     "deferred query count out of range");
+        }
+        NBX_TRY(per_query_walk(B<int32_t>(s.deferred), static_cast<int>(counts[2])));
+        NBX_HIP(error, hipEventRecord(e->ev[6], st));
+        NBX_HIP(error, hipMemcpyAsync(counts, s.counters.p, 2 * sizeof(unsigned long long),
+                                      hipMemcpyDeviceToHost, st));
+        NBX_HIP(error, hipStreamSynchronize(st));
+    }
     e->stats.accepted_approximations = counts[0];
     e->stats.exact_interactions = counts[1];
     return N_BODY_SIM_PRO_STATUS_OK;

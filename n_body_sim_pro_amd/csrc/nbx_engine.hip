@@ -78,7 +78,7 @@ auto work_buffers(Engine* e) {
             &s.hist, &s.mperm, &s.mkey_sorted, &s.tperm, &s.trank, &s.tword, &s.ext, &s.delta,
             &s.cnt, &s.scan, &s.scan_part, &s.sp4, &s.s4, &s.tile_hist, &s.tile_base, &s.n64,
             &s.n32, &s.icom, &s.irec, &s.idepth, &s.gnode, &s.im2, &s.gquad,
-            &s.tmp32, &s.counters, &e->scratch, &e->scratch2};
+            &s.tmp32, &s.counters, &s.deferred, &e->scratch, &e->scratch2};
 }
 
 void free_buf(Buf& b) {
@@ -711,6 +711,8 @@ NBodySimProStatus nbx_engine_bh_export_nodes(NBXEngine* engine, double* rows,
         }
     }
     {
+        NBX_TRY(nbx::bh_node_records(e, error));
+        NBX_HIP(error, hipStreamSynchronize(e->stream));
         std::vector<nbx::NodeF32> recs(nodes);
         NBX_HIP(error, hipMemcpy(recs.data(), s.n32.p, nodes * sizeof(nbx::NodeF32),
                                  hipMemcpyDeviceToHost));

@@ -103,6 +103,7 @@ struct BHState {
     bool mismatch_path = false;
     bool has_n64 = false;     // the last build wrote FP64 node records
     bool has_quad = false;    // the last build wrote quadrupole records
+    bool has_n32 = false;     // the FP32 pre-order records of the last build are written
     const uint64_t* ext_tree = nullptr;   // tree-order extension words (tie path) or null
     double root[4] = {0, 0, 0, 1};
     // pinned host block the build reads its flags, per-depth counts and root
@@ -128,8 +129,8 @@ struct BHState {
     Buf s4;                   // tree-order FP32 x, y, z, m
     Buf tile_hist, tile_base; // per-tile, per-depth internal-node counts and their scan
     // nodes (pre-order, children in descending octant order)
-    Buf n64;                  // NodeF64
-    Buf n32;                  // NodeF32
+    Buf n64;                  // NodeF64 (FP64 builds)
+    Buf n32;                  // NodeF32, written on demand (bh_node_records)
     // internal cells by level index (depth-major, tree order within a depth):
     // FP64 centre of mass and mass (double4), (node index, skip, first and
     // last tree position) (int4), depth, GroupNode (FP32 record and child
@@ -141,7 +142,8 @@ struct BHState {
     // records), learned path only
     Buf im2, gquad;
     Buf tmp32;                // scratch permutation
-    Buf counters;             // unsigned long long [approx, exact]
+    Buf counters;             // unsigned long long [approx, exact, deferred queries]
+    Buf deferred;             // tree positions the group walk leaves to the per-query walk
 };
 
 struct Engine {
@@ -178,6 +180,9 @@ NBodySimProStatus barnes_hut(Engine* e, const NBodySimProGravity* g, bool fp64_e
                              NBodySimProError* error);
 NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error,
                            bool quadrupoles = false);
+// The FP32 pre-order node records of the last build (NodeF32, for the
+// per-query walks and the node export), written once per build on first use.
+NBodySimProStatus bh_node_records(Engine* e, NBodySimProError* error);
 // One walk per bh_build: the build clears the walk's interaction counters.
 NBodySimProStatus bh_walk(Engine* e, const NBodySimProGravity* g, bool fp64_exact,
                           float* features, NBodySimProError* error);
