@@ -48,6 +48,7 @@
 #include "nbx_internal.h"
 
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 namespace nbx {
@@ -187,36 +188,47 @@ __device__ inline int run_start(const uint64_t* w, const uint64_t* ext, int t, i
 }
 
 // ---------------------------------------------------------------- inputs and bounding cube
-// Build inputs as one (x, y, z, m) record per particle: FP32-rounded and
-// widened back to FP64 (hip_barnes_hut) or the FP64 values (the FP64 walk).
-template <bool ROUND>
-__device__ inline double4 build_input(const double* __restrict__ px, const double* __restrict__ py,
-                                      const double* __restrict__ pz, const double* __restrict__ m,
-                                      int i) {
-    if (ROUND) {
-        return make_double4(static_cast<double>(static_cast<float>(px[i])),
-                            static_cast<double>(static_cast<float>(py[i])),
-                            static_cast<double>(static_cast<float>(pz[i])),
-                            static_cast<double>(static_cast<float>(m[i])));
+// Build inputs as one (x, y, z, m) record per particle (type In): the
+// FP32-rounded values as float4 (hip_barnes_hut; every FP64 operation of the
+// build sees them widened back, exactly) or the FP64 values as double4 (the
+// FP64 walk). Half the bytes for the FP32 build in every pass that moves them.
+__device__ inline double4 widen(const double4& v) {
+    return v;
+}
+
+__device__ inline double4 widen(const float4& v) {
+    return make_double4(static_cast<double>(v.x), static_cast<double>(v.y),
+                        static_cast<double>(v.z), static_cast<double>(v.w));
+}
+
+template <typename In>
+__device__ inline In build_input(const double* __restrict__ px, const double* __restrict__ py,
+                                 const double* __restrict__ pz, const double* __restrict__ m,
+                                 int i) {
+    if constexpr (std::is_same_v<In, float4>) {
+        return make_float4(static_cast<float>(px[i]), static_cast<float>(py[i]),
+                           static_cast<float>(pz[i]), static_cast<float>(m[i]));
+    } else {
+        return make_double4(px[i], py[i], pz[i], m[i]);
     }
-    return make_double4(px[i], py[i], pz[i], m[i]);
 }
 
 // barnes_hut.c:453-483. min/max are exact, so the reduction order is free.
 // The same grid-stride pass writes the build inputs.
 constexpr int kBBoxBlocks = 512;
 
-template <bool ROUND>
+template <typename In>
 __global__ __launch_bounds__(kThreads) void bbox_partial_kernel(
     const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
-    const double* __restrict__ m, double4* __restrict__ b4, int n, double* __restrict__ partial) {
+    const double* __restrict__ m, In* __restrict__ b4, int n, double* __restrict__ partial) {
     __shared__ double s[6][kThreads];
-    const double4 p0 = build_input<ROUND>(px, py, pz, m, 0);
+    const double4 p0 = widen(build_input<In>(px, py, pz, m, 0));
     double lo[3] = {p0.x, p0.y, p0.z};
     double hi[3] = {lo[0], lo[1], lo[2]};
     for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
-        const double4 q = build_input<ROUND>(px, py, pz, m, i);
-        b4[i] = q;
+        const In v = build_input<In>(px, py, pz, m, i);
+        b4[i] = v;
+        const double4 q = widen(v);
         const double p[3] = {q.x, q.y, q.z};
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
@@ -308,7 +320,8 @@ __global__ __launch_bounds__(kThreads) void bbox_final_kernel(const double* __re
 }
 
 // ---------------------------------------------------------------- keys
-__global__ void keys_kernel(const double4* __restrict__ b4, int n, const double* __restrict__ root,
+template <typename In>
+__global__ void keys_kernel(const In* __restrict__ b4, int n, const double* __restrict__ root,
                             uint64_t* __restrict__ mkey, uint64_t* __restrict__ tkey,
                             uint64_t* __restrict__ sortkey, uint32_t* __restrict__ idx,
                             unsigned* __restrict__ flags) {
@@ -317,7 +330,7 @@ __global__ void keys_kernel(const double4* __restrict__ b4, int n, const double*
         return;
     }
     const double cx = root[0], cy = root[1], cz = root[2], h = root[3];
-    const double4 p = b4[i];
+    const double4 p = widen(b4[i]);
     const double x = p.x, y = p.y, z = p.z;
     const uint64_t mk = spread3(axis_code(x, cx, h)) | (spread3(axis_code(y, cy, h)) << 1) |
                         (spread3(axis_code(z, cz, h)) << 2);
@@ -333,14 +346,15 @@ __global__ void keys_kernel(const double4* __restrict__ b4, int n, const double*
 }
 
 // ext[pos*6+k] = octant digits of levels 21(k+1) .. 21(k+1)+20 of particle orig[pos].
-__global__ void ext_words_kernel(const double4* __restrict__ b4, int n,
+template <typename In>
+__global__ void ext_words_kernel(const In* __restrict__ b4, int n,
                                  const double* __restrict__ root,
                                  const uint32_t* __restrict__ orig, uint64_t* __restrict__ ext) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) {
         return;
     }
-    const double4 q = b4[orig[p]];
+    const double4 q = widen(b4[orig[p]]);
     const double x = q.x, y = q.y, z = q.z;
     double cx = root[0], cy = root[1], cz = root[2], h = root[3];
     path_word(x, y, z, cx, cy, cz, h);
@@ -381,13 +395,15 @@ __global__ void iota_kernel(uint32_t* __restrict__ out, int n) {
 // Without asc_rank the order is the Morton order and, unless the keys_kernel
 // flagged a true-path key that differs from its Morton key (then this pass is
 // redone with asc_rank), the true keys are the sorted Morton keys: read in
-// order instead of gathered.
+// order instead of gathered. The tree-order inputs: FP32 s4 always, FP64 sp4
+// for FP64 inputs only (an FP32 build widens s4 where it needs FP64).
+template <typename In>
 __global__ void tree_order_kernel(const uint32_t* __restrict__ asc_rank,
                                   const uint32_t* __restrict__ mperm,
                                   const uint64_t* __restrict__ mkey_sorted,
                                   const uint64_t* __restrict__ tkey_orig,
                                   const uint64_t* __restrict__ ext_asc,
-                                  const double4* __restrict__ b4, int n, uint32_t* __restrict__ tperm, uint32_t* __restrict__ trank,
+                                  const In* __restrict__ b4, int n, uint32_t* __restrict__ tperm, uint32_t* __restrict__ trank,
                                   uint64_t* __restrict__ tword, uint64_t* __restrict__ ext_tree,
                                   double4* __restrict__ sp4, float4* __restrict__ s4) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -406,10 +422,14 @@ __global__ void tree_order_kernel(const uint32_t* __restrict__ asc_rank,
                 ext_asc[static_cast<size_t>(k) * kExtWords + w];
         }
     }
-    const double4 p = b4[o];
-    sp4[t] = p;
-    s4[t] = make_float4(static_cast<float>(p.x), static_cast<float>(p.y), static_cast<float>(p.z),
-                        static_cast<float>(p.w));
+    const In p = b4[o];
+    if constexpr (std::is_same_v<In, float4>) {
+        s4[t] = p;
+    } else {
+        sp4[t] = p;
+        s4[t] = make_float4(static_cast<float>(p.x), static_cast<float>(p.y),
+                            static_cast<float>(p.z), static_cast<float>(p.w));
+    }
 }
 
 // ---------------------------------------------------------------- topology
@@ -768,12 +788,12 @@ __global__ __launch_bounds__(kTopoTile) void fill_kernel(
 // an internal cell (level index L, its FP64 centre of mass and mass, half
 // size h, pre-order skip). Only the per-query walks read these records, so
 // node_records_kernel writes them on demand after the build.
-__device__ inline NodeF32 leaf_record(const double4& q, int li, int t) {
+__device__ inline NodeF32 leaf_record(const float4& q, int li, int t) {
     NodeF32 b;
-    b.cx = static_cast<float>(q.x);
-    b.cy = static_cast<float>(q.y);
-    b.cz = static_cast<float>(q.z);
-    b.m = static_cast<float>(q.w);
+    b.cx = q.x;
+    b.cy = q.y;
+    b.cz = q.z;
+    b.m = q.w;
     b.size2 = 0.f;
     b.skip = li + 1;
     b.leaf = t;
@@ -799,11 +819,12 @@ __device__ inline NodeF32 cell_record(const double4& c, double h, int skip, int 
 // particle that created it (lowest Morton rank in the cell; for the root the
 // caller's particle 0, barnes_hut.c:504-512). Children are one level deeper
 // and were finished by the previous launch (or barrier). A child is one
-// 32-byte record: the tree-order particle or the child's centre of mass.
-template <bool QUAD>
+// record: the tree-order particle (In: FP32 inputs as float4, widened) or the
+// child's FP64 centre of mass.
+template <bool QUAD, typename In>
 __device__ inline void com_node(int L, int depth, GroupNode* __restrict__ gnode,
                                 InternalSoA in, const uint32_t* __restrict__ trank,
-                                const double4* __restrict__ sp4, const double4* __restrict__ b4,
+                                const In* __restrict__ sp4, const In* __restrict__ b4,
                                 double root_h, NodeF64* __restrict__ n64,
                                 double* __restrict__ im2, float4* __restrict__ gquad) {
     const int4 r = in.rec[L];
@@ -818,7 +839,7 @@ __device__ inline void com_node(int L, int depth, GroupNode* __restrict__ gnode,
         if (c == -1) {
             continue;
         }
-        const double4 q = c < -1 ? sp4[-c - 2] : in.com[c];
+        const double4 q = c < -1 ? widen(sp4[-c - 2]) : in.com[c];
         mass += q.w;
         x += q.w * q.x;
         y += q.w * q.y;
@@ -830,7 +851,7 @@ __device__ inline void com_node(int L, int depth, GroupNode* __restrict__ gnode,
         oy = y / mass;
         oz = z / mass;
     } else if (p == 0) {
-        const double4 p0 = b4[0];
+        const double4 p0 = widen(b4[0]);
         ox = p0.x;
         oy = p0.y;
         oz = p0.z;
@@ -845,7 +866,7 @@ __device__ inline void com_node(int L, int depth, GroupNode* __restrict__ gnode,
                 minpos = t;
             }
         }
-        const double4 q = sp4[minpos];
+        const double4 q = widen(sp4[minpos]);
         ox = q.x;
         oy = q.y;
         oz = q.z;
@@ -875,7 +896,7 @@ __device__ inline void com_node(int L, int depth, GroupNode* __restrict__ gnode,
                 if (c == -1) {
                     continue;
                 }
-                const double4 q = c < -1 ? sp4[-c - 2] : in.com[c];
+                const double4 q = c < -1 ? widen(sp4[-c - 2]) : in.com[c];
                 const double dx = q.x - ox, dy = q.y - oy, dz = q.z - oz;
                 xx += q.w * dx * dx;
                 yy += q.w * dy * dy;
@@ -913,25 +934,25 @@ __device__ inline void com_node(int L, int depth, GroupNode* __restrict__ gnode,
 }
 
 // One depth level per launch (large levels): cells first .. first + count - 1.
-template <bool QUAD>
+template <bool QUAD, typename In>
 __global__ __launch_bounds__(kThreads) void com_level_kernel(
     int first, int count, int depth, GroupNode* __restrict__ gnode,
-    InternalSoA in, const uint32_t* __restrict__ trank, const double4* __restrict__ sp4,
-    const double4* __restrict__ b4, const double* __restrict__ root, NodeF64* __restrict__ n64,
+    InternalSoA in, const uint32_t* __restrict__ trank, const In* __restrict__ sp4,
+    const In* __restrict__ b4, const double* __restrict__ root, NodeF64* __restrict__ n64,
     double* __restrict__ im2, float4* __restrict__ gquad) {
     const int k = blockIdx.x * kThreads + threadIdx.x;
     if (k < count) {
-        com_node<QUAD>(first + k, depth, gnode, in, trank, sp4, b4, root[3], n64, im2, gquad);
+        com_node<QUAD, In>(first + k, depth, gnode, in, trank, sp4, b4, root[3], n64, im2, gquad);
     }
 }
 
 // Consecutive small depth levels (deepest first) in one workgroup; the
 // barrier orders each level's results before the next level reads them.
-template <bool QUAD>
+template <bool QUAD, typename In>
 __global__ __launch_bounds__(1024) void com_small_levels_kernel(
     const unsigned* __restrict__ dcount, int d_hi, int d_lo, GroupNode* __restrict__ gnode,
-    InternalSoA in, const uint32_t* __restrict__ trank, const double4* __restrict__ sp4,
-    const double4* __restrict__ b4, const double* __restrict__ root, NodeF64* __restrict__ n64,
+    InternalSoA in, const uint32_t* __restrict__ trank, const In* __restrict__ sp4,
+    const In* __restrict__ b4, const double* __restrict__ root, NodeF64* __restrict__ n64,
     double* __restrict__ im2, float4* __restrict__ gquad) {
     __shared__ unsigned doff[kMaxLevels];
     depth_offsets(dcount, doff);
@@ -941,7 +962,7 @@ __global__ __launch_bounds__(1024) void com_small_levels_kernel(
         const int count = static_cast<int>(dcount[d]);
         const int first = static_cast<int>(doff[d]);
         for (int k = threadIdx.x; k < count; k += blockDim.x) {
-            com_node<QUAD>(first + k, d, gnode, in, trank, sp4, b4, root_h, n64, im2, gquad);
+            com_node<QUAD, In>(first + k, d, gnode, in, trank, sp4, b4, root_h, n64, im2, gquad);
         }
         __syncthreads();
     }
@@ -950,15 +971,16 @@ __global__ __launch_bounds__(1024) void com_small_levels_kernel(
 // The FP32 pre-order records of every node, for the per-query walks, from a
 // finished build: thread k < n writes the leaf at tree position k (node index
 // scan[k] + cnt[k] + k), thread n + L the internal cell of level index L (node
-// index rec[L].x). The values are those the build's own FP64 records round to.
+// index rec[L].x). The values are the build's own FP64 values rounded to FP32
+// (the leaves' are the tree-order s4).
 __global__ __launch_bounds__(kThreads) void node_records_kernel(
     int n, int internal, const uint32_t* __restrict__ scan, const uint32_t* __restrict__ cnt,
-    const double4* __restrict__ sp4, InternalSoA in, const double* __restrict__ root,
+    const float4* __restrict__ s4, InternalSoA in, const double* __restrict__ root,
     NodeF32* __restrict__ n32) {
     const int k = blockIdx.x * kThreads + threadIdx.x;
     if (k < n) {
         const int li = static_cast<int>(scan[k] + cnt[k]) + k;
-        n32[li] = leaf_record(sp4[k], li, k);
+        n32[li] = leaf_record(s4[k], li, k);
     } else if (k - n < internal) {
         const int L = k - n;
         const int4 r = in.rec[L];
@@ -1785,8 +1807,11 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error,
     const unsigned g = grid_for(n, kThreads);
 
     NBX_HIP(error, hipEventRecord(e->ev[0], st));
-    NBX_TRY(s.b4.need<double4>(n, error));
-    double4* b4 = B<double4>(s.b4);
+    // build inputs: float4 for FP32-rounded inputs, double4 for FP64 ones
+    NBX_TRY(s.b4.need<char>(n * (fp32_inputs ? sizeof(float4) : sizeof(double4)), error));
+    float4* b4f = B<float4>(s.b4);
+    double4* b4d = B<double4>(s.b4);
+    s.inputs_fp64 = !fp32_inputs;
     const unsigned bb_blocks = g < static_cast<unsigned>(kBBoxBlocks) ? g : kBBoxBlocks;
     NBX_TRY(s.partial.need<double>(6 * kBBoxBlocks + 8, error));
     NBX_TRY(s.flags.need<unsigned>(kFlagWords, error));
@@ -1805,7 +1830,9 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error,
     NBX_TRY(s.delta.need<int32_t>(n, error));
     NBX_TRY(s.cnt.need<uint32_t>(n, error));
     NBX_TRY(s.scan.need<uint32_t>(n + 1, error));
-    NBX_TRY(s.sp4.need<double4>(n, error));
+    if (!fp32_inputs) {
+        NBX_TRY(s.sp4.need<double4>(n, error));
+    }
     NBX_TRY(s.s4.need<float4>(n, error));
     NBX_TRY(s.counters.need<unsigned long long>(3, error));
     NBX_TRY(s.deferred.need<int32_t>(n, error));
@@ -1817,18 +1844,24 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error,
     double* root = B<double>(s.partial) + 6 * kBBoxBlocks;
     unsigned* flags = B<unsigned>(s.flags);
     if (fp32_inputs) {
-        bbox_partial_kernel<true><<<bb_blocks, kThreads, 0, st>>>(e->px, e->py, e->pz, e->m, b4,
-                                                                  ni, B<double>(s.partial));
+        bbox_partial_kernel<float4><<<bb_blocks, kThreads, 0, st>>>(e->px, e->py, e->pz, e->m,
+                                                                    b4f, ni, B<double>(s.partial));
     } else {
-        bbox_partial_kernel<false><<<bb_blocks, kThreads, 0, st>>>(e->px, e->py, e->pz, e->m, b4,
-                                                                   ni, B<double>(s.partial));
+        bbox_partial_kernel<double4><<<bb_blocks, kThreads, 0, st>>>(e->px, e->py, e->pz, e->m,
+                                                                     b4d, ni, B<double>(s.partial));
     }
     bbox_final_kernel<<<1, kThreads, 0, st>>>(B<double>(s.partial), static_cast<int>(bb_blocks),
                                               root, flags, kFlagWords,
                                               B<unsigned long long>(s.counters));
-    keys_kernel<<<g, kThreads, 0, st>>>(b4, ni, root, B<uint64_t>(s.mkey),
-                                        B<uint64_t>(s.tkey), B<uint64_t>(s.sk0),
-                                        B<uint32_t>(s.sv0), flags);
+    if (fp32_inputs) {
+        keys_kernel<float4><<<g, kThreads, 0, st>>>(b4f, ni, root, B<uint64_t>(s.mkey),
+                                                    B<uint64_t>(s.tkey), B<uint64_t>(s.sk0),
+                                                    B<uint32_t>(s.sv0), flags);
+    } else {
+        keys_kernel<double4><<<g, kThreads, 0, st>>>(b4d, ni, root, B<uint64_t>(s.mkey),
+                                                     B<uint64_t>(s.tkey), B<uint64_t>(s.sk0),
+                                                     B<uint32_t>(s.sv0), flags);
+    }
     NBX_HIP(error, hipGetLastError());
     NBX_HIP(error, hipEventRecord(e->ev[1], st));
 
@@ -1849,11 +1882,17 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error,
     double* hroot = reinterpret_cast<double*>(hflags + kFlagRead);
     auto topology = [&](const uint32_t* asc_rank, const uint64_t* ext_asc, uint64_t* ext_t)
         -> NBodySimProStatus {
-        tree_order_kernel<<<g, kThreads, 0, st>>>(
-            asc_rank, B<uint32_t>(s.mperm), B<uint64_t>(s.mkey_sorted), B<uint64_t>(s.tkey),
-            ext_asc, b4, ni,
-            B<uint32_t>(s.tperm), B<uint32_t>(s.trank), B<uint64_t>(s.tword), ext_t,
-            B<double4>(s.sp4), B<float4>(s.s4));
+        if (fp32_inputs) {
+            tree_order_kernel<float4><<<g, kThreads, 0, st>>>(
+                asc_rank, B<uint32_t>(s.mperm), B<uint64_t>(s.mkey_sorted), B<uint64_t>(s.tkey),
+                ext_asc, b4f, ni, B<uint32_t>(s.tperm), B<uint32_t>(s.trank),
+                B<uint64_t>(s.tword), ext_t, nullptr, B<float4>(s.s4));
+        } else {
+            tree_order_kernel<double4><<<g, kThreads, 0, st>>>(
+                asc_rank, B<uint32_t>(s.mperm), B<uint64_t>(s.mkey_sorted), B<uint64_t>(s.tkey),
+                ext_asc, b4d, ni, B<uint32_t>(s.tperm), B<uint32_t>(s.trank),
+                B<uint64_t>(s.tword), ext_t, B<double4>(s.sp4), B<float4>(s.s4));
+        }
         delta_kernel<<<nt, kThreads, 0, st>>>(B<uint64_t>(s.tword), ext_t, ni, nt,
                                               B<int32_t>(s.delta), B<uint32_t>(s.cnt), flags,
                                               B<unsigned>(s.tile_hist));
@@ -1910,7 +1949,13 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error,
         uint64_t* ext_m = B<uint64_t>(s.ext);
         uint64_t* ext_a = ext_m + n * kExtWords;
         uint64_t* ext_t = ext_a + n * kExtWords;
-        ext_words_kernel<<<g, kThreads, 0, st>>>(b4, ni, root, B<uint32_t>(s.mperm), ext_m);
+        if (fp32_inputs) {
+            ext_words_kernel<float4><<<g, kThreads, 0, st>>>(b4f, ni, root, B<uint32_t>(s.mperm),
+                                                             ext_m);
+        } else {
+            ext_words_kernel<double4><<<g, kThreads, 0, st>>>(b4d, ni, root,
+                                                              B<uint32_t>(s.mperm), ext_m);
+        }
         uint32_t* P = B<uint32_t>(s.tmp32);
         iota_kernel<<<g, kThreads, 0, st>>>(P, ni);
         for (int k = kExtWords - 1; k >= 0; --k) {
@@ -1974,7 +2019,9 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error,
         s.has_n64 = true;
     }
     const InternalSoA in = internal_soa(s);
-    const double4* sp4 = B<double4>(s.sp4);
+    // tree-order FP64 inputs (FP64 builds; FP32 builds widen s4)
+    const double4* sp4 = fp32_inputs ? nullptr : B<double4>(s.sp4);
+    const float4* s4 = B<float4>(s.s4);
     s.ext_tree = ext_tree;
     const unsigned* start_base = B<unsigned>(s.tile_base);
     const unsigned* end_base = start_base + static_cast<size_t>(kMaxLevels) * tiles;
@@ -1997,20 +2044,58 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error,
     // MI355X, 1M two-galaxy: small levels up to 512 cells 0.113 ms for the
     // whole pass, up to 1,536 cells 0.119 ms, up to 2,560 cells 0.136 ms.)
     constexpr unsigned kSmallLevel = 512;
+    // the four centre-of-mass variants: quadrupole moments or not, FP32 or
+    // FP64 particle inputs
+    auto com_level = [&](int d, unsigned count) {
+        const unsigned grid = grid_for(count, kThreads);
+        const int first = static_cast<int>(offs[d]);
+        const int c = static_cast<int>(count);
+        GroupNode* gn = B<GroupNode>(s.gnode);
+        const uint32_t* tr = B<uint32_t>(s.trank);
+        if (fp32_inputs) {
+            if (quadrupoles) {
+                com_level_kernel<true, float4><<<grid, kThreads, 0, st>>>(
+                    first, c, d, gn, in, tr, s4, b4f, root, n64_out, im2, gquad);
+            } else {
+                com_level_kernel<false, float4><<<grid, kThreads, 0, st>>>(
+                    first, c, d, gn, in, tr, s4, b4f, root, n64_out, nullptr, nullptr);
+            }
+        } else {
+            if (quadrupoles) {
+                com_level_kernel<true, double4><<<grid, kThreads, 0, st>>>(
+                    first, c, d, gn, in, tr, sp4, b4d, root, n64_out, im2, gquad);
+            } else {
+                com_level_kernel<false, double4><<<grid, kThreads, 0, st>>>(
+                    first, c, d, gn, in, tr, sp4, b4d, root, n64_out, nullptr, nullptr);
+            }
+        }
+    };
+    auto com_small_levels = [&](int d_hi, int d_lo) {
+        GroupNode* gn = B<GroupNode>(s.gnode);
+        const uint32_t* tr = B<uint32_t>(s.trank);
+        const unsigned* dc = flags + kDCount;
+        if (fp32_inputs) {
+            if (quadrupoles) {
+                com_small_levels_kernel<true, float4><<<1, 1024, 0, st>>>(
+                    dc, d_hi, d_lo, gn, in, tr, s4, b4f, root, n64_out, im2, gquad);
+            } else {
+                com_small_levels_kernel<false, float4><<<1, 1024, 0, st>>>(
+                    dc, d_hi, d_lo, gn, in, tr, s4, b4f, root, n64_out, nullptr, nullptr);
+            }
+        } else {
+            if (quadrupoles) {
+                com_small_levels_kernel<true, double4><<<1, 1024, 0, st>>>(
+                    dc, d_hi, d_lo, gn, in, tr, sp4, b4d, root, n64_out, im2, gquad);
+            } else {
+                com_small_levels_kernel<false, double4><<<1, 1024, 0, st>>>(
+                    dc, d_hi, d_lo, gn, in, tr, sp4, b4d, root, n64_out, nullptr, nullptr);
+            }
+        }
+    };
     for (int d = deepest; d >= 0;) {
         const unsigned count = hflags[kDCount + d];
         if (count > kSmallLevel) {
-            if (quadrupoles) {
-                com_level_kernel<true><<<grid_for(count, kThreads), kThreads, 0, st>>>(
-                    static_cast<int>(offs[d]), static_cast<int>(count), d,
-                    B<GroupNode>(s.gnode), in, B<uint32_t>(s.trank), sp4, b4, root, n64_out,
-                    im2, gquad);
-            } else {
-                com_level_kernel<false><<<grid_for(count, kThreads), kThreads, 0, st>>>(
-                    static_cast<int>(offs[d]), static_cast<int>(count), d,
-                    B<GroupNode>(s.gnode), in, B<uint32_t>(s.trank), sp4, b4, root, n64_out,
-                    nullptr, nullptr);
-            }
+            com_level(d, count);
             --d;
             continue;
         }
@@ -2018,16 +2103,7 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error,
         while (lo - 1 >= 0 && hflags[kDCount + lo - 1] <= kSmallLevel) {
             --lo;
         }
-        if (quadrupoles) {
-            com_small_levels_kernel<true><<<1, 1024, 0, st>>>(
-                flags + kDCount, d, lo, B<GroupNode>(s.gnode), in,
-                B<uint32_t>(s.trank), sp4, b4, root, n64_out, im2, gquad);
-        } else {
-            com_small_levels_kernel<false><<<1, 1024, 0, st>>>(
-                flags + kDCount, d, lo, B<GroupNode>(s.gnode), in,
-                B<uint32_t>(s.trank), sp4, b4, root, n64_out, nullptr,
-                nullptr);
-        }
+        com_small_levels(d, lo);
         d = lo - 1;
     }
     NBX_HIP(error, hipGetLastError());
@@ -2059,7 +2135,7 @@ NBodySimProStatus bh_node_records(Engine* e, NBodySimProError* error) {
     const int ni = static_cast<int>(s.n);
     const int ic = static_cast<int>(s.internal);
     node_records_kernel<<<grid_for(s.nodes, kThreads), kThreads, 0, e->stream>>>(
-        ni, ic, B<uint32_t>(s.scan), B<uint32_t>(s.cnt), B<double4>(s.sp4), internal_soa(s),
+        ni, ic, B<uint32_t>(s.scan), B<uint32_t>(s.cnt), B<float4>(s.s4), internal_soa(s),
         B<double>(s.partial) + 6 * kBBoxBlocks, B<NodeF32>(s.n32));
     NBX_HIP(error, hipGetLastError());
     s.has_n32 = true;

@@ -725,7 +725,14 @@ NBodySimProStatus nbx_engine_bh_export_nodes(NBXEngine* engine, double* rows,
     std::vector<int32_t> delta(n);
     std::vector<double> sp(4 * n);
     NBX_HIP(error, hipMemcpy(delta.data(), s.delta.p, n * 4, hipMemcpyDeviceToHost));
-    NBX_HIP(error, hipMemcpy(sp.data(), s.sp4.p, n * 32, hipMemcpyDeviceToHost));
+    if (s.inputs_fp64) {
+        NBX_HIP(error, hipMemcpy(sp.data(), s.sp4.p, n * 32, hipMemcpyDeviceToHost));
+    } else {
+        // FP32 inputs: the tree-order float4 values, widened exactly
+        std::vector<float> sf(4 * n);
+        NBX_HIP(error, hipMemcpy(sf.data(), s.s4.p, n * 16, hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < 4 * n; ++k) sp[k] = static_cast<double>(sf[k]);
+    }
     for (size_t k = 0; k < nodes; ++k) {
         const int t = leaf[k];
         if (t < 0) {

@@ -104,12 +104,14 @@ struct BHState {
     bool has_n64 = false;     // the last build wrote FP64 node records
     bool has_quad = false;    // the last build wrote quadrupole records
     bool has_n32 = false;     // the FP32 pre-order records of the last build are written
+    bool inputs_fp64 = false; // the last build ran on FP64 inputs (b4 and sp4 are double4)
     const uint64_t* ext_tree = nullptr;   // tree-order extension words (tie path) or null
     double root[4] = {0, 0, 0, 1};
     // pinned host block the build reads its flags, per-depth counts and root
     // cube into (asynchronous copies; owned here so they always land)
     void* host_flags = nullptr;
-    // build inputs (FP64 x, y, z, m): either the master arrays or FP32-rounded copies
+    // build inputs (x, y, z, m): FP32-rounded copies of the master arrays
+    // (float4) or the FP64 values (double4)
     Buf b4;                   // (x, y, z, m) per particle in original order
     Buf partial;              // bbox partials + root
     Buf flags;                // flags, per-depth counts, offsets and cursors
@@ -125,7 +127,7 @@ struct BHState {
     Buf delta;                // int32 common levels between tree positions t, t+1
     Buf cnt, scan;            // internal nodes started per tree position, exclusive scan
     Buf scan_part;            // scan block partials
-    Buf sp4;                  // tree-order FP64 (x, y, z, m)
+    Buf sp4;                  // tree-order FP64 (x, y, z, m), FP64 builds only
     Buf s4;                   // tree-order FP32 x, y, z, m
     Buf tile_hist, tile_base; // per-tile, per-depth internal-node counts and their scan
     // nodes (pre-order, children in descending octant order)
