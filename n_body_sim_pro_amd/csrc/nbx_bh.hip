@@ -130,63 +130,6 @@ __device__ inline int digit_at(const uint64_t* __restrict__ w, const uint64_t* _
     return static_cast<int>((word >> (60 - 3 * (level % 21))) & 7u);
 }
 
-// Last position e >= t with common_levels(t, e) >= d (galloping, then binary
-// search: deep cells are short runs, so most searches end after a few probes).
-__device__ inline int run_end(const uint64_t* w, const uint64_t* ext, int n, int t, int d) {
-    int lo = t, step = 1;
-    int hi = n - 1;
-    while (true) {
-        const int probe = t + step;
-        if (probe >= n) {
-            break;
-        }
-        if (common_levels(w, ext, t, probe) >= d) {
-            lo = probe;
-            step <<= 1;
-        } else {
-            hi = probe - 1;
-            break;
-        }
-    }
-    while (lo < hi) {
-        const int mid = lo + (hi - lo + 1) / 2;
-        if (common_levels(w, ext, t, mid) >= d) {
-            lo = mid;
-        } else {
-            hi = mid - 1;
-        }
-    }
-    return lo;
-}
-
-// First position s <= t with common_levels(s, t) >= d (galloping backwards).
-__device__ inline int run_start(const uint64_t* w, const uint64_t* ext, int t, int d) {
-    int hi = t, step = 1;
-    int lo = 0;
-    while (true) {
-        const int probe = t - step;
-        if (probe < 0) {
-            break;
-        }
-        if (common_levels(w, ext, probe, t) >= d) {
-            hi = probe;
-            step <<= 1;
-        } else {
-            lo = probe + 1;
-            break;
-        }
-    }
-    while (lo < hi) {
-        const int mid = lo + (hi - lo) / 2;
-        if (common_levels(w, ext, mid, t) >= d) {
-            hi = mid;
-        } else {
-            lo = mid + 1;
-        }
-    }
-    return lo;
-}
-
 // ---------------------------------------------------------------- inputs and bounding cube
 // Build inputs as one (x, y, z, m) record per particle (type In): the
 // FP32-rounded values as float4 (hip_barnes_hut; every FP64 operation of the
@@ -1329,10 +1272,44 @@ constexpr int kGroupLeaves = 96;     // staged leaves per wave
 constexpr int kGroupStack = 512;     // pending cells per wave
 constexpr int kRangeStack = 32;      // pending sub-ranges per wave
 constexpr int kMaxSubWalks = 16;     // shared-list walks per wave
-#ifndef NBX_GROUP_CELL
-#define NBX_GROUP_CELL 2048
-#endif
-constexpr int kGroupCell = NBX_GROUP_CELL;   // compact: the group's smallest enclosing cell holds <= kGroupCell Q
+constexpr int kGroupCell = 2048;     // compact: the group's smallest enclosing cell holds <= kGroupCell
+
+// Whether the run of tree positions that share the first c levels with a and
+// b (a < b, which share exactly c; tree order is key order, so the run is an
+// interval) holds at most kGroupCell positions. Each lane probes one position
+// past each end at a stride of kGroupCell / 64 (the in-run probes are a prefix
+// of the lanes), then half the lanes probe each end's last stride position by
+// position: two rounds of independent loads instead of two galloping binary
+// searches of up to ~40 dependent loads.
+constexpr int kRunStride = kGroupCell / kWave;
+static_assert(kRunStride == kWave / 2, "the fine round probes one stride per half wave");
+
+__device__ inline bool run_compact(const uint64_t* __restrict__ w, const uint64_t* __restrict__ ext,
+                                   int n, int a, int b, int c, int lane) {
+    const int r = b + kRunStride * (lane + 1);
+    const int l = a - kRunStride * (lane + 1);
+    const bool in_r = r < n && common_levels(w, ext, b, r) >= c;
+    const bool in_l = l >= 0 && common_levels(w, ext, l, a) >= c;
+    const int kr = __popcll(__builtin_amdgcn_ballot_w64(in_r));
+    const int kl = __popcll(__builtin_amdgcn_ballot_w64(in_l));
+    if (kr == kWave || kl == kWave) {
+        return false;   // the run reaches kGroupCell positions past one end
+    }
+    // lanes 0..30 probe the 31 positions after b + stride * kr, lanes 32..62
+    // the 31 before a - stride * kl
+    const int f = lane & (kRunStride - 1);
+    const bool right = lane < kRunStride;
+    const int p = right ? b + kRunStride * kr + f + 1 : a - kRunStride * kl - f - 1;
+    bool in = false;
+    if (f < kRunStride - 1) {
+        in = right ? p < n && common_levels(w, ext, b, p) >= c
+                   : p >= 0 && common_levels(w, ext, p, a) >= c;
+    }
+    const uint64_t fine = __builtin_amdgcn_ballot_w64(in);
+    const int last = b + kRunStride * kr + __popcll(fine & 0xFFFFFFFFull);
+    const int first = a - kRunStride * kl - __popcll(fine >> 32);
+    return last - first + 1 <= kGroupCell;
+}
 #ifndef NBX_GROUP_THREADS
 #define NBX_GROUP_THREADS 64
 #endif
@@ -1681,12 +1658,9 @@ __global__ __launch_bounds__(kGroupThreads) void group_walk_kernel(
         const int a = __builtin_amdgcn_readfirstlane(r.x);
         const int b = __builtin_amdgcn_readfirstlane(r.y);
         wave_lds_fence();   // the pop is read before a push reuses the slot
-        const int c = a < b ? common_levels(tword, ext, a, b) : max_level;
-        bool shared = c >= max_level;
-        if (!shared) {
-            const int count = run_end(tword, ext, n, b, c) - run_start(tword, ext, a, c) + 1;
-            shared = count <= kGroupCell;
-        }
+        const int c =
+            __builtin_amdgcn_readfirstlane(a < b ? common_levels(tword, ext, a, b) : max_level);
+        const bool shared = c >= max_level || run_compact(tword, ext, n, a, b, c, lane);
         uint64_t cut = 0;
         if (!shared && rsp + 8 <= kRangeStack && walks + rsp + 2 <= max_walks) {
             // octant of each member's depth c + 1 cell; runs of equal octants
