@@ -1313,6 +1313,21 @@ __device__ inline bool run_compact(const uint64_t* __restrict__ w, const uint64_
 #ifndef NBX_GROUP_THREADS
 #define NBX_GROUP_THREADS 64
 #endif
+// Groups are dealt to the XCDs in chunks of kGroupChunk consecutive groups
+// (chunk c to XCD c mod 8, each XCD's chunks in tree order): every XCD gets a
+// share of the dense and the sparse regions, and a chunk's waves still share
+// cell records in their XCD's L2. (Measured on MI355X, two-galaxy: with each
+// XCD on one contiguous eighth of the groups, the XCDs finished between 630
+// and 846 us at 1M and between 12.8 and 16.2 ms at 10M, theta = 0.5; walk
+// 0.78 / 16.4 ms. Chunks of 8, 32, 128 groups: 0.80 / 14.4, 0.80 / 14.2,
+// 0.78 / 14.2 ms.)
+constexpr int kGroupChunk = 128;
+
+__device__ inline int group_of_block(int b) {
+    const int slot = b / kXcds;   // workgroup b runs on XCD b mod 8
+    return ((slot / kGroupChunk) * kXcds + b % kXcds) * kGroupChunk + slot % kGroupChunk;
+}
+
 // One wave per workgroup: a wave's LDS is released as soon as it finishes
 // (waves of one region take very different times; measured on MI355X, 1M
 // particles: 1.45 ms with one wave per workgroup, 1.66 ms with four).
@@ -1408,7 +1423,8 @@ __global__ __launch_bounds__(kGroupThreads) void group_walk_kernel(
     float4* lleaf = lleaf_all[wid];
     unsigned char* selfhit = selfhit_all[wid];
     int2* ranges = ranges_all[wid];
-    const int wave = xcd_block(blockIdx.x, gridDim.x) * kWaves + wid;
+    static_assert(kWaves == 1, "one group per workgroup");
+    const int wave = group_of_block(static_cast<int>(blockIdx.x));
     const int t0 = wave * kWave;
     if (t0 >= n) {
         return;   // whole wave beyond the particles
@@ -2178,8 +2194,11 @@ NBodySimProStatus bh_walk(Engine* e, const NBodySimProGravity* g, bool fp64_exac
         // theta = 1.0, 10M: 9.61 -> 7.96 ms.)
         const bool member_test =
             features != nullptr || (e->flags & NBX_FLAG_BH_BOX_TEST) == 0;
-        const unsigned blocks_g = grid_for(s.n, kGroupThreads);
-        const unsigned grid_g = (blocks_g + kXcds - 1) / kXcds * kXcds;
+        // whole rounds of one chunk per XCD (blocks past the last group exit)
+        const size_t chunks = (s.n + static_cast<size_t>(kWave) * kGroupChunk - 1) /
+                              (static_cast<size_t>(kWave) * kGroupChunk);
+        const unsigned grid_g =
+            static_cast<unsigned>((chunks + kXcds - 1) / kXcds * kXcds * kGroupChunk);
         if (features != nullptr) {
             group_walk_kernel<true><<<grid_g, kGroupThreads, 0, st>>>(
                 deferred, gn, ni_cells, B<float4>(s.s4), B<uint32_t>(s.tperm),
