@@ -57,7 +57,7 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kExtWords = 6;
 constexpr int kMaxLevels = 129;  // a shared depth-129 cell means depth > 128 (barnes_hut.c:149-153)
-enum { FLAG_MISMATCH = 0, FLAG_TIE = 1, FLAG_OVERFLOW = 2 };
+enum { FLAG_MISMATCH = 0, FLAG_TIE = 1, FLAG_OVERFLOW = 2, FLAG_LONG_RUN = 3 };
 
 __device__ inline uint64_t spread3(uint64_t a) {
     uint64_t x = a & 0x1FFFFFull;
@@ -286,6 +286,58 @@ __global__ void keys_kernel(const In* __restrict__ b4, int n, const double* __re
     if (mk != tk) {
         atomicOr(&flags[FLAG_MISMATCH], 1u);
     }
+}
+
+// The Morton sort runs its radix passes on the top kSortTopBits key bits
+// only (levels 0..11: four of the seven 9-bit passes); sort_finish_kernel
+// then orders each run of equal top bits by the low kSortLowBits, stably.
+// The runs are short (two-galaxy input: at most 10 particles at 1M, 53 at
+// 10M). A run longer than kFinishRun raises FLAG_LONG_RUN, and the host sorts
+// again on all 63 bits after reading the flags.
+constexpr int kSortLowBits = 27;
+constexpr int kSortTopBits = 63 - kSortLowBits;
+constexpr int kFinishRun = 64;
+static_assert(kSortTopBits % 9 == 0, "whole 9-bit radix passes");
+
+// Input: sorted (stably, from index order) by key >> kSortLowBits. Element p
+// goes to its run's first position plus the number of run members before it
+// in (low bits, position) order. Elements of a run longer than kFinishRun
+// stay where they are (the output is still a permutation) and raise the flag.
+__global__ __launch_bounds__(kThreads) void sort_finish_kernel(
+    const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+    uint64_t* __restrict__ kout, uint32_t* __restrict__ vout, int n,
+    unsigned* __restrict__ flags) {
+    const int p = blockIdx.x * kThreads + threadIdx.x;
+    if (p >= n) {
+        return;
+    }
+    const uint64_t key = kin[p];
+    const uint64_t top = key >> kSortLowBits;
+    const uint64_t low = key & ((1ull << kSortLowBits) - 1);
+    // the run's bounds, at most kFinishRun steps to each side
+    int a = p;
+    while (a > 0 && p - a < kFinishRun && (kin[a - 1] >> kSortLowBits) == top) {
+        --a;
+    }
+    int b = p;
+    while (b < n - 1 && b - p < kFinishRun && (kin[b + 1] >> kSortLowBits) == top) {
+        ++b;
+    }
+    int dst = p;
+    if (b - a + 1 > kFinishRun) {
+        // every member of a run this long sees a window of more than
+        // kFinishRun positions, so the whole run stays in place
+        atomicOr(&flags[FLAG_LONG_RUN], 1u);
+    } else {
+        int rank = 0;
+        for (int q = a; q <= b; ++q) {
+            const uint64_t lq = kin[q] & ((1ull << kSortLowBits) - 1);
+            rank += (lq < low || (lq == low && q < p)) ? 1 : 0;
+        }
+        dst = a + rank;
+    }
+    kout[dst] = key;
+    vout[dst] = vin[p];
 }
 
 // ext[pos*6+k] = octant digits of levels 21(k+1) .. 21(k+1)+20 of particle orig[pos].
@@ -1855,10 +1907,17 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error,
     NBX_HIP(error, hipGetLastError());
     NBX_HIP(error, hipEventRecord(e->ev[1], st));
 
-    // Morton order: stable sort of (key, original index) on the 63 key bits.
+    // Morton order: stable sort of (key, original index) on the top key bits
+    // (the result stays in sk0/sv0), then each run of equal top bits ordered
+    // by the low bits. (Measured on MI355X, two-galaxy, against seven passes
+    // on all 63 bits: sort 1M 0.171 -> 0.108 ms, 10M 0.894 -> 0.641 ms.)
     NBX_TRY(radix_sort_pairs(st, B<uint64_t>(s.sk0), B<uint32_t>(s.sv0), B<uint64_t>(s.sk1),
-                             B<uint32_t>(s.sv1), n, 63, s.hist, error,
-                             B<uint64_t>(s.mkey_sorted), B<uint32_t>(s.mperm)));
+                             B<uint32_t>(s.sv1), n, kSortTopBits, s.hist, error, nullptr,
+                             nullptr, kSortLowBits));
+    sort_finish_kernel<<<g, kThreads, 0, st>>>(B<uint64_t>(s.sk0), B<uint32_t>(s.sv0),
+                                               B<uint64_t>(s.mkey_sorted), B<uint32_t>(s.mperm),
+                                               ni, flags);
+    NBX_HIP(error, hipGetLastError());
     NBX_HIP(error, hipEventRecord(e->ev[9], st));
 
     // Tree order, neighbour depths and per-depth node counts for a given
@@ -1917,6 +1976,19 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error,
     NBX_TRY(topology(nullptr, nullptr, nullptr));
     s.mismatch_path = hflags[FLAG_MISMATCH] != 0;
     s.tie_path = false;
+    if (hflags[FLAG_LONG_RUN] != 0) {
+        // a run of equal top key bits too long for sort_finish_kernel (the
+        // Morton order above was wrong there): sort on all 63 bits, then
+        // redo the topology
+        NBX_HIP(error, hipMemcpyAsync(s.sk0.p, s.mkey.p, n * sizeof(uint64_t),
+                                      hipMemcpyDeviceToDevice, st));
+        iota_kernel<<<g, kThreads, 0, st>>>(B<uint32_t>(s.sv0), ni);
+        NBX_TRY(radix_sort_pairs(st, B<uint64_t>(s.sk0), B<uint32_t>(s.sv0), B<uint64_t>(s.sk1),
+                                 B<uint32_t>(s.sv1), n, 63, s.hist, error,
+                                 B<uint64_t>(s.mkey_sorted), B<uint32_t>(s.mperm)));
+        NBX_HIP(error, hipMemsetAsync(flags, 0, kFlagWords * sizeof(unsigned), st));
+        NBX_TRY(topology(nullptr, nullptr, nullptr));
+    }
     if (s.mismatch_path) {
         // asc_rank[k]: Morton rank of the k-th particle in ascending true-path order
         gather_u64_kernel<<<g, kThreads, 0, st>>>(B<uint64_t>(s.tkey), B<uint32_t>(s.mperm),

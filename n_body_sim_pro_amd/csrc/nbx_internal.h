@@ -191,13 +191,15 @@ NBodySimProStatus bh_walk(Engine* e, const NBodySimProGravity* g, bool fp64_exac
 NBodySimProStatus bh_learned(Engine* e, const NBodySimProGravity* g, NBodySimProError* error);
 
 // Stable LSD radix sort of (64-bit key, 32-bit value) pairs, ascending key;
-// equal keys keep their input order. `bits` = number of low key bits to sort.
-// The result goes to keys_out/vals_out when given (they must not alias the
-// alternate buffers), else back into keys/vals.
+// equal keys keep their input order. It sorts by key bits first_bit ..
+// first_bit + bits - 1 (the bits below first_bit are ignored: equal there
+// counts as equal). The result goes to keys_out/vals_out when given (they
+// must not alias the alternate buffers), else back into keys/vals.
 NBodySimProStatus radix_sort_pairs(hipStream_t s, uint64_t* keys, uint32_t* vals,
                                    uint64_t* keys_alt, uint32_t* vals_alt, size_t n, int bits,
                                    Buf& hist, NBodySimProError* error,
-                                   uint64_t* keys_out = nullptr, uint32_t* vals_out = nullptr);
+                                   uint64_t* keys_out = nullptr, uint32_t* vals_out = nullptr,
+                                   int first_bit = 0);
 // Exclusive scan of uint32 (total written to out[n]).
 NBodySimProStatus exclusive_scan_u32(hipStream_t s, const uint32_t* in, uint32_t* out, size_t n,
                                      Buf& partial, NBodySimProError* error);

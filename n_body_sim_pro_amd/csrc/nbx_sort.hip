@@ -312,7 +312,7 @@ __global__ __launch_bounds__(kScanThreads) void scan_apply(const uint32_t* __res
 NBodySimProStatus radix_sort_pairs(hipStream_t s, uint64_t* keys, uint32_t* vals,
                                    uint64_t* keys_alt, uint32_t* vals_alt, size_t n, int bits,
                                    Buf& hist, NBodySimProError* error, uint64_t* keys_out,
-                                   uint32_t* vals_out) {
+                                   uint32_t* vals_out, int first_bit) {
     if (keys_out == nullptr || vals_out == nullptr) {
         keys_out = keys;
         vals_out = vals;
@@ -339,7 +339,7 @@ NBodySimProStatus radix_sort_pairs(hipStream_t s, uint64_t* keys, uint32_t* vals
     const uint64_t* kin = keys;
     const uint32_t* vin = vals;
     for (int p = 0; p < passes; ++p) {
-        const int shift = kDigitBits * p;
+        const int shift = first_bit + kDigitBits * p;
         // ping-pong between the two buffers; the last pass writes the output
         const bool last = p == passes - 1;
         uint64_t* kout = last ? keys_out : ((p & 1) ? keys : keys_alt);
