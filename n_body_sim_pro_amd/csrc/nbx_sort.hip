@@ -5,7 +5,9 @@
 // stable 21-bit LSD passes and a 2^21-entry counting array
 // (src/c/barnes_hut/barnes_hut.c:312-339). Any stable sort on the full key
 // yields the same permutation (ties keep ascending original index). Here the
-// digit is 9 bits (7 passes cover 63 bits) and a pass is three launches:
+// digit is 9 bits (7 passes cover 63 bits; a caller may sort a window of the
+// key bits, as the Morton sort does with the top 36) and a pass is three
+// launches:
 //   sort_histogram   per-tile digit counts (2048-item tiles, 4096 for large
 //                    inputs; LDS atomics);
 //   sort_scan_digit  per digit, exclusive scan over tiles (+ digit totals);
