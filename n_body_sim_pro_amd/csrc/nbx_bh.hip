@@ -2104,7 +2104,9 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error,
     // Deepest level first. Runs of small levels share one single-workgroup
     // launch; a large level gets a full-grid launch of its own. (Measured on
     // MI355X, 1M two-galaxy: small levels up to 512 cells 0.113 ms for the
-    // whole pass, up to 1,536 cells 0.119 ms, up to 2,560 cells 0.136 ms.)
+    // whole pass, up to 1,536 cells 0.119 ms, up to 2,560 cells 0.136 ms.
+    // Two adjacent large levels per launch, each workgroup computing its
+    // cells' children first: 0.100 -> 0.114 ms at 1M, 0.40 -> 0.45 ms at 10M.)
     constexpr unsigned kSmallLevel = 512;
     // the four centre-of-mass variants: quadrupole moments or not, FP32 or
     // FP64 particle inputs

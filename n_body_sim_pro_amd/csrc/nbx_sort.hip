@@ -262,22 +262,29 @@ __global__ __launch_bounds__(kScanThreads) void scan_tiles(const uint32_t* __res
     }
 }
 
+// One workgroup: each thread sums a contiguous chunk of the tile totals, one
+// block scan of the chunk sums, then each thread writes its chunk's running
+// offsets. (Measured on MI355X: 9,766 tiles at 10M particles took 42 us as
+// one block scan per 256 totals.)
 __global__ __launch_bounds__(kScanThreads) void scan_partials(uint32_t* __restrict__ partial,
                                                              unsigned count) {
     __shared__ uint32_t s[kScanThreads];
-    uint32_t carry = 0;
-    for (unsigned start = 0; start < count; start += kScanThreads) {
-        const unsigned i = start + threadIdx.x;
-        const uint32_t v = i < count ? partial[i] : 0u;
-        uint32_t total;
-        const uint32_t ex = block_exclusive_scan(v, s, &total);
-        if (i < count) {
-            partial[i] = carry + ex;
-        }
-        carry += total;
+    const unsigned per = (count + kScanThreads - 1) / kScanThreads;
+    const unsigned lo = min(count, threadIdx.x * per);
+    const unsigned hi = min(count, lo + per);
+    uint32_t sum = 0;
+    for (unsigned i = lo; i < hi; ++i) {
+        sum += partial[i];
+    }
+    uint32_t total;
+    uint32_t run = block_exclusive_scan(sum, s, &total);
+    for (unsigned i = lo; i < hi; ++i) {
+        const uint32_t v = partial[i];
+        partial[i] = run;
+        run += v;
     }
     if (threadIdx.x == 0) {
-        partial[count] = carry;
+        partial[count] = total;
     }
 }
 
