@@ -159,6 +159,32 @@ def test_bh_two_galaxy_fp32_inputs_keys_order_topology(nbx, oracle):
     assert abs((s_t[3] + s_t[4]) - (o_t[3] + o_t[4])) <= 1e-3 * (o_t[3] + o_t[4])
 
 
+@pytest.mark.parametrize("case", ["lattice", "outlier", "massless_region"])
+def test_bh_fp32_inputs_crafted_keys_order_topology(nbx, oracle, case):
+    """hip_barnes_hut (FP32-rounded inputs, float4 build inputs) on the
+    crafted inputs: keys, sorted order, root cube and node rows bit-exact
+    against the oracle's build on the same rounded inputs, and the default
+    group walk close to the reference walk. The cluster plus far outlier puts
+    500 particles in one run of equal top 36 key bits, more than the Morton
+    sort's finish kernel takes: that build sorts again on all 63 bits."""
+    pos, m = crafted_inputs()[case]
+    raw = Particles(len(m), {"px": pos[:, 0], "py": pos[:, 1], "pz": pos[:, 2], "m": m})
+    pos32 = pos.astype(np.float32).astype(np.float64)
+    m32 = m.astype(np.float32).astype(np.float64)
+    p32 = Particles(len(m), {"px": pos32[:, 0], "py": pos32[:, 1], "pz": pos32[:, 2], "m": m32})
+    st, oacc, os_, okeys, operm, oroot, orows = oracle_bh(oracle, p32, 0.7, 0.01)
+    assert st == 0
+    e, acc, s = gpu_bh(nbx, raw, 0.7, 0.01, "hip_barnes_hut")
+    root, keys, perm = e.bh_morton()
+    rows = e.bh_export_nodes()
+    e.close()
+    assert np.array_equal(root, oroot)
+    assert np.array_equal(keys, okeys)
+    assert np.array_equal(perm, operm)
+    assert np.array_equal(sort_rows(rows)[:, :7], sort_rows(orows)[:, :7])
+    assert rms_rel_err(acc, oacc) < 0.05
+
+
 @pytest.mark.parametrize("n", [200_000, 4_500_000])
 def test_group_walk_refines_reference_lists(nbx, oracle, n):
     """The FP32 group walk accepts a cell only when the whole group's box
