@@ -1990,15 +1990,33 @@ NBodySimProStatus bh_build(Engine* e, bool fp32_inputs, NBodySimProError* error,
         NBX_TRY(topology(nullptr, nullptr, nullptr));
     }
     if (s.mismatch_path) {
-        // asc_rank[k]: Morton rank of the k-th particle in ascending true-path order
+        // asc_rank[k]: Morton rank of the k-th particle in ascending true-path
+        // order (ties keep Morton order): the true keys in Morton order,
+        // sorted like the Morton keys (top bits, then the short runs).
         gather_u64_kernel<<<g, kThreads, 0, st>>>(B<uint64_t>(s.tkey), B<uint32_t>(s.mperm),
                                                   B<uint64_t>(s.sk0), B<uint32_t>(s.sv0), ni, 1, 0);
         NBX_TRY(radix_sort_pairs(st, B<uint64_t>(s.sk0), B<uint32_t>(s.sv0), B<uint64_t>(s.sk1),
-                                 B<uint32_t>(s.sv1), n, 63, s.hist, error));
-        NBX_HIP(error, hipMemcpyAsync(s.tmp32.p, s.sv0.p, n * sizeof(uint32_t),
-                                      hipMemcpyDeviceToDevice, st));
+                                 B<uint32_t>(s.sv1), n, kSortTopBits, s.hist, error, nullptr,
+                                 nullptr, kSortLowBits));
         NBX_HIP(error, hipMemsetAsync(flags, 0, kFlagWords * sizeof(unsigned), st));
+        sort_finish_kernel<<<g, kThreads, 0, st>>>(B<uint64_t>(s.sk0), B<uint32_t>(s.sv0),
+                                                   B<uint64_t>(s.sk1), B<uint32_t>(s.tmp32), ni,
+                                                   flags);
+        NBX_HIP(error, hipGetLastError());
         NBX_TRY(topology(B<uint32_t>(s.tmp32), nullptr, nullptr));
+        if (hflags[FLAG_LONG_RUN] != 0) {
+            // a run of equal top bits too long for the finish kernel: all 63 bits
+            gather_u64_kernel<<<g, kThreads, 0, st>>>(B<uint64_t>(s.tkey), B<uint32_t>(s.mperm),
+                                                      B<uint64_t>(s.sk0), B<uint32_t>(s.sv0), ni,
+                                                      1, 0);
+            NBX_TRY(radix_sort_pairs(st, B<uint64_t>(s.sk0), B<uint32_t>(s.sv0),
+                                     B<uint64_t>(s.sk1), B<uint32_t>(s.sv1), n, 63, s.hist,
+                                     error));
+            NBX_HIP(error, hipMemcpyAsync(s.tmp32.p, s.sv0.p, n * sizeof(uint32_t),
+                                          hipMemcpyDeviceToDevice, st));
+            NBX_HIP(error, hipMemsetAsync(flags, 0, kFlagWords * sizeof(unsigned), st));
+            NBX_TRY(topology(B<uint32_t>(s.tmp32), nullptr, nullptr));
+        }
     }
 
     const uint64_t* ext_tree = nullptr;

@@ -94,6 +94,11 @@ def crafted_inputs():
     X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
     pos = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
     out["lattice"] = (pos, np.ones(len(pos)))
+    # the lattice plus 100 particles within 1e-6 of one point: one run of
+    # equal top 36 key bits longer than the Morton sort's finish kernel takes,
+    # next to the lattice's points on cell-centre planes
+    cluster = np.array([0.3, 0.3, 0.3]) + rng.uniform(-1e-6, 1e-6, size=(100, 3))
+    out["lattice_cluster"] = (np.concatenate([pos, cluster]), np.ones(len(pos) + 100))
     # pairs closer than one 21-bit Morton cell (tie path: deeper than 21 levels)
     base = rng.uniform(-1, 1, size=(300, 3))
     close = base + rng.uniform(-1, 1, size=(300, 3)) * 1e-9
@@ -114,7 +119,7 @@ def crafted_inputs():
 
 
 @pytest.mark.parametrize("case", ["lattice", "close_pairs", "outlier", "massless_region",
-                                  "massless_all"])
+                                  "massless_all", "lattice_cluster"])
 @pytest.mark.parametrize("theta", [0.3, 0.7])
 def test_bh_fp64_crafted_inputs_bit_identical(nbx, oracle, case, theta):
     pos, m = crafted_inputs()[case]
@@ -159,7 +164,7 @@ def test_bh_two_galaxy_fp32_inputs_keys_order_topology(nbx, oracle):
     assert abs((s_t[3] + s_t[4]) - (o_t[3] + o_t[4])) <= 1e-3 * (o_t[3] + o_t[4])
 
 
-@pytest.mark.parametrize("case", ["lattice", "outlier", "massless_region"])
+@pytest.mark.parametrize("case", ["lattice", "outlier", "massless_region", "lattice_cluster"])
 def test_bh_fp32_inputs_crafted_keys_order_topology(nbx, oracle, case):
     """hip_barnes_hut (FP32-rounded inputs, float4 build inputs) on the
     crafted inputs: keys, sorted order, root cube and node rows bit-exact
