@@ -281,6 +281,28 @@ def test_group_walk_features_are_consistent(nbx):
     assert not f[:, 10:].any()
 
 
+def test_feature_walk_fallback_is_the_reference_list_walk(nbx):
+    """With the split off (NBX_FLAG_BH_NO_SPLIT) the feature walk leaves the
+    particles of non-compact groups to the per-query walk that runs after
+    it: their raw features are exactly the reference-list engine's, every
+    other particle's exactly the default feature walk's."""
+    n = 200_000
+    d = nbx.two_galaxy(n, seed=42)
+    out = {}
+    for key, kw in (("default", {}), ("no_split", {"split": False}),
+                    ("reference", {"reference_lists": True})):
+        with nbx.Engine(n, theta=1.0, **kw) as e:
+            e.upload(d["pos"], d["m"])
+            out[key] = e.bh_features(1.0, 0.01)
+    assert all(np.isfinite(f).all() for f in out.values())
+    same_group = np.all(out["no_split"] == out["default"], axis=1)
+    same_ref = np.all(out["no_split"] == out["reference"], axis=1)
+    assert np.all(same_group | same_ref)
+    frac = 1.0 - same_group.mean()
+    print(f"particles on the per-query feature walk: {frac:.3%}")
+    assert 0.02 < frac < 0.3
+
+
 def test_correction_kernel_matches_host_formula(nbx, learned):
     d = nbx.plummer(20000, seed=12)
     p, mean, std = _random_model(7)
